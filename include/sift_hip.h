@@ -1,0 +1,168 @@
+/*
+ * sift_hip.h -- C ABI of libsift_hip.so, the MI355X (gfx950) SIFT backend.
+ *
+ * This is the drop-in boundary for the reference's SIFT path. A host binding
+ * (the Rust `SiftBackend::Hip` arm, a ctypes wrapper, C++) calls these entry
+ * points; no C++ exception or Rust panic crosses them. Return codes: 0 = OK,
+ * negative = error, with a thread-local message in sift_hip_last_error().
+ *
+ * Reference interfaces replaced (stevel705/sift-wgpu):
+ *   sift_keypoint_t      <- KeyPoint                      src/keypoints.rs:1-10
+ *   sift_params_t        <- Sift {..}, Sift::new          src/sift.rs:36-80
+ *   sift_hip_detect      <- Sift::detect_and_compute_cpu  src/sift.rs:1200-1240
+ *                           via detect_and_compute_with_backend src/sift.rs:1244-1355
+ *   sift_hip_pyramid     <- generate_gaussian_pyramid + generate_dog_pyramid
+ *                           (with the initial blur)        src/sift.rs:129-204,1205-1214
+ *   sift_hip_detect_stages <- find_scale_space_extrema / refine_and_filter_extrema
+ *                           stage outputs                  src/sift.rs:510-603,299-474
+ */
+#ifndef SIFT_HIP_H
+#define SIFT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes. */
+#define SIFT_OK 0
+#define SIFT_ERR_INVALID -1   /* bad argument (null pointer, zero size, bad params) */
+#define SIFT_ERR_DEVICE -2    /* HIP runtime error (no device, launch failure, ...) */
+#define SIFT_ERR_NOMEM -3     /* host or device allocation failed */
+#define SIFT_ERR_BUSY -4      /* serving queue full / admission refused */
+#define SIFT_ERR_TIMEOUT -5   /* sift_hip_poll timed out with nothing completed */
+
+/* KeyPoint (src/keypoints.rs:1-10), #[repr(C)] mirror: 5 x f32 + 2 x i32 = 28 B.
+ * x, y in input-image pixels; size = sigma; angle in radians, (-pi, pi]. */
+typedef struct sift_keypoint_t {
+    float x, y, size, angle, response;
+    int32_t octave, layer;
+} sift_keypoint_t;
+
+/* Fields of `Sift` (src/sift.rs:36-45). contrast_threshold is the STORED value,
+ * i.e. already divided by num_intervals as Sift::new does (src/sift.rs:76);
+ * pass `sift.contrast_threshold` through unchanged. image_border_width is 5 in
+ * the reference (src/sift.rs:17,78). */
+typedef struct sift_params_t {
+    float sigma;
+    uint32_t num_octaves;
+    uint32_t num_intervals;
+    float assumed_blur;
+    float contrast_threshold;
+    float edge_threshold;
+    uint32_t image_border_width;
+} sift_params_t;
+
+/* 8-bit grey image, row-major. stride = bytes between rows (>= width; 0 means
+ * width). For the *_device entry points `data` is a device pointer. */
+typedef struct sift_image_t {
+    const uint8_t* data;
+    uint32_t width, height, stride;
+} sift_image_t;
+
+/* Host result, allocated by the library, released with sift_result_free.
+ * descriptors: count x 128 f32, row-major, same convention as the CPU path
+ * (normalised, clipped at 0.2, renormalised; src/sift.rs:889-913). Order: as the
+ * CPU path emits them (octave, layer, row, column of the extremum, then
+ * orientation peaks in ascending bin order). */
+typedef struct sift_result_t {
+    sift_keypoint_t* keypoints;
+    float* descriptors;
+    size_t count;
+} sift_result_t;
+
+typedef struct sift_hip_ctx sift_hip_ctx;
+
+/* Sift::default() values (src/sift.rs:47-60): sigma 1.6, 4 octaves, 3 intervals,
+ * assumed_blur 0.5, contrast 0.04/3 (stored), edge 10, border 5. */
+void sift_params_default(sift_params_t* out);
+
+/* Sift::new(sigma, octaves, intervals, assumed_blur, contrast, edge)
+ * (src/sift.rs:63-80): stores contrast / intervals. */
+void sift_params_new(sift_params_t* out, float sigma, uint32_t num_octaves,
+                     uint32_t num_intervals, float assumed_blur, float contrast_threshold,
+                     float edge_threshold);
+
+/* Creates a context on HIP device `device` (ordinal). Fails with
+ * SIFT_ERR_DEVICE when no device is visible. One stream per context; calls on
+ * one context are serialised internally. */
+int sift_hip_create(const sift_params_t* params, int device, sift_hip_ctx** out);
+void sift_hip_destroy(sift_hip_ctx* ctx);
+
+/* Thread-local message for the last failing call on this thread ("" if none). */
+const char* sift_hip_last_error(void);
+
+/* One image, host in / host out (the `SiftBackend::Hip` arm's call). */
+int sift_hip_detect(sift_hip_ctx* ctx, const sift_image_t* image, sift_result_t* out);
+
+/* n images of any sizes in one pass (all stages batched across images).
+ * params: NULL = the context's params for every image, else n entries.
+ * out: n results. */
+int sift_hip_detect_batch(sift_hip_ctx* ctx, const sift_image_t* images,
+                          const sift_params_t* params, size_t n, sift_result_t* out);
+
+/* Device-resident variant: `images[i].data` are device pointers; results stay
+ * in device memory owned by the context and valid until the next call on it.
+ * out_counts (host, n entries, may be NULL) receive per-image keypoint counts
+ * after the call returns (it synchronises once). d_keypoints/d_descriptors
+ * (may be NULL) receive device pointers to the concatenated results in image
+ * order; d_offsets (may be NULL) a device pointer to n+1 uint32 offsets. */
+int sift_hip_detect_batch_device(sift_hip_ctx* ctx, const sift_image_t* images,
+                                 const sift_params_t* params, size_t n, size_t* out_counts,
+                                 const sift_keypoint_t** d_keypoints,
+                                 const float** d_descriptors, const uint32_t** d_offsets);
+
+void sift_result_free(sift_result_t* r);
+
+/* Pyramid introspection: initial blur + Gaussian pyramid + DoG pyramid
+ * (src/sift.rs:1205-1214,129-204) for one image, copied to host.
+ * gauss: sum_o (S+3)*W_o*H_o bytes; dog: sum_o (S+2)*W_o*H_o floats (may be NULL);
+ * octave-major, level-major, row-major, no padding. W_{o+1} = floor(W_o/2). */
+int sift_hip_pyramid(sift_hip_ctx* ctx, const sift_image_t* image, uint8_t* gauss, float* dog);
+
+/* Stage outputs for parity checks: extremum candidates (pre-refinement,
+ * src/sift.rs:510-603) and refined keypoints (src/sift.rs:299-474, before
+ * orientation), plus the final result. Arrays are library-allocated and freed
+ * by sift_stages_free. */
+typedef struct sift_stages_t {
+    sift_keypoint_t* candidates;
+    size_t n_candidates;
+    sift_keypoint_t* refined;
+    size_t n_refined;
+    sift_result_t result;
+} sift_stages_t;
+
+int sift_hip_detect_stages(sift_hip_ctx* ctx, const sift_image_t* image, sift_stages_t* out);
+void sift_stages_free(sift_stages_t* s);
+
+/* ---- Serving: continuous batching on one device -------------------------
+ * Frames are admitted while device memory for their pyramids fits within
+ * `hbm_budget_bytes`; frames arriving while others run are batched into the
+ * next launch, so their work overlaps on the device. Completed frames are
+ * returned by sift_hip_poll with the submit tag. */
+typedef struct sift_server_config_t {
+    size_t hbm_budget_bytes;  /* device bytes the in-flight pyramid pool may use */
+    uint32_t max_batch;       /* max frames per device launch (0 = 64) */
+    uint32_t max_queue;       /* max queued (not yet admitted) frames (0 = 4096) */
+} sift_server_config_t;
+
+typedef struct sift_completion_t {
+    uint64_t tag;
+    int status;               /* SIFT_OK or an error code */
+    sift_result_t result;     /* owned by the caller after poll; free with sift_result_free */
+    double submit_ms, start_ms, done_ms;  /* steady-clock ms, for latency accounting */
+} sift_completion_t;
+
+int sift_hip_server_start(sift_hip_ctx* ctx, const sift_server_config_t* cfg);
+/* Copies the pixels before returning. */
+int sift_hip_submit(sift_hip_ctx* ctx, const sift_image_t* image, uint64_t tag);
+int sift_hip_poll(sift_hip_ctx* ctx, sift_completion_t* out, size_t cap, size_t* n,
+                  int timeout_ms);
+int sift_hip_server_stop(sift_hip_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,948 @@
+// context.hip -- host side of libsift_hip.so: parameter math, device memory,
+// batch planning, the launch sequence and the C ABI of include/sift_hip.h.
+//
+// Host f32 arithmetic here mirrors the reference's operation order (compiled
+// with -ffp-contract=off, SSE f32, glibc libm): level sigmas
+// (src/sift.rs:132-147), initial blur (src/sift.rs:1205-1209), imageproc's
+// Gaussian taps and image's Lanczos3 weights. These feed the kernels as tables.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "sift_hip.h"
+#include "sift_internal.h"
+#include "sift_context.h"
+
+using namespace sh;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(SIFT_ERR_DEVICE, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr float kPiF = 3.14159274101257324219f;
+
+size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// f32::powi with a run-time exponent (compiler-rt __powisf2: square-and-multiply).
+float powi_f(float a, int b) {
+    int recip = b < 0;
+    float r = 1.0f;
+    for (;;) {
+        if (b & 1) r *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
+    }
+    return recip ? 1.0f / r : r;
+}
+
+// imageproc gaussian_kernel_f32: r = ceil(2 sigma); taps = (sqrt(2 pi) sigma)^-1 *
+// exp(-i^2 / (2 sigma^2)); not normalised.
+int gaussian_taps(float sigma, std::vector<float>& taps) {
+    int r = (int)ceilf(2.0f * sigma);
+    int n = 2 * r + 1;
+    taps.assign(n, 0.0f);
+    float norm = 1.0f / (sqrtf(2.0f * kPiF) * sigma);
+    float two_s2 = 2.0f * (sigma * sigma);
+    for (int i = 0; i <= r; i++) {
+        float x = (float)i;
+        float v = norm * expf(-(x * x) / two_s2);
+        taps[r + i] = v;
+        taps[r - i] = v;
+    }
+    return n;
+}
+
+// image 0.25.6 Lanczos3: sinc(t) = sin(t*PI)/(t*PI), window 3*sratio, weights
+// divided by their f32 sum.
+float sinc_f(float t) {
+    float a = t * kPiF;
+    if (t == 0.0f) return 1.0f;
+    return sinf(a) / a;
+}
+float lanczos3_f(float x) { return fabsf(x) < 3.0f ? sinc_f(x) * sinc_f(x / 3.0f) : 0.0f; }
+
+}  // namespace
+
+namespace sh {
+
+// ---------------------------------------------------------------------------
+// Table store: taps and Lanczos tables in one growing device buffer.
+// ---------------------------------------------------------------------------
+int TableStore::ensure_uploaded() {
+    if (host.size() == uploaded) return SIFT_OK;
+    if (host.size() > dev_cap) {
+        size_t ncap = std::max(host.size() * 2, (size_t)1 << 20);
+        void* nd = nullptr;
+        HIP_TRY(hipMalloc(&nd, ncap));
+        if (dev) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(dev);
+        }
+        dev = (char*)nd;
+        dev_cap = ncap;
+        uploaded = 0;
+    }
+    HIP_TRY(hipMemcpy(dev + uploaded, host.data() + uploaded, host.size() - uploaded,
+                      hipMemcpyHostToDevice));
+    uploaded = host.size();
+    return SIFT_OK;
+}
+
+size_t TableStore::append(const void* p, size_t bytes) {
+    size_t off = round_up(host.size(), 256);
+    host.resize(off + bytes);
+    memcpy(host.data() + off, p, bytes);
+    return off;
+}
+
+const ParamTables& Context::tables_for(const sift_params_t& p) {
+    ParamKey key{p.sigma, p.assumed_blur, p.num_intervals, p.num_octaves};
+    auto it = param_tables.find(key);
+    if (it != param_tables.end()) return it->second;
+    ParamTables T;
+    const int S = (int)p.num_intervals, L = S + 3, O = (int)p.num_octaves;
+    auto add_taps = [&](float sigma, TapsRef& out) {
+        std::vector<float> taps;
+        int n = gaussian_taps(sigma, taps);
+        int npad = (int)round_up((size_t)n, kTapBlock);
+        taps.resize(npad, 0.0f);
+        out.ntaps = n;
+        out.npad = npad;
+        // 2.05 (n+2) u bounds 2 g(n+1) / (1 - g(n)) with margin (u = 2^-24).
+        out.eps_rel = (float)(2.05 * (double)(n + 2) * 5.9604644775390625e-8);
+        out.off = tables.append(taps.data(), taps.size() * sizeof(float));
+        out.copy = false;
+    };
+    // initial blur (src/sift.rs:1205-1214)
+    float s0 = 0.0f;
+    if (p.sigma > p.assumed_blur) s0 = sqrtf(p.sigma * p.sigma - p.assumed_blur * p.assumed_blur);
+    if (s0 > 1e-4f) add_taps(s0, T.initial);
+    else T.initial.copy = true;
+    // level sigmas (src/sift.rs:132-147)
+    float k = powf(2.0f, 1.0f / (float)S);
+    std::vector<float> tgt(L);
+    for (int s = 0; s < L; s++) tgt[s] = p.sigma * powi_f(k, s);
+    T.level.resize((size_t)O * L);
+    for (int o = 0; o < O; o++) {
+        float prev = p.sigma * powi_f(2.0f, o);
+        for (int s = 1; s < L; s++) {
+            float t = tgt[s] * powi_f(2.0f, o);
+            float b = sqrtf(t * t - prev * prev);
+            TapsRef& R = T.level[(size_t)o * L + s];
+            if (b < 1e-4f || !(b == b)) R.copy = true;  // src/sift.rs:149-150
+            else add_taps(b, R);
+            prev = t;
+        }
+    }
+    return param_tables.emplace(key, std::move(T)).first->second;
+}
+
+const LanczosRef& Context::lanczos_for(int in, int out) {
+    auto key = std::make_pair(in, out);
+    auto it = lanczos.find(key);
+    if (it != lanczos.end()) return it->second;
+    float ratio = (float)in / (float)out;
+    float sratio = ratio < 1.0f ? 1.0f : ratio;
+    float support = 3.0f * sratio;
+    std::vector<int> left(out), cnt(out);
+    int maxt = 0;
+    for (int j = 0; j < out; j++) {
+        float c = ((float)j + 0.5f) * ratio;
+        long long l = (long long)floorf(c - support);
+        l = std::min(std::max(l, 0LL), (long long)in - 1);
+        long long r = (long long)ceilf(c + support);
+        r = std::min(std::max(r, l + 1), (long long)in);
+        left[j] = (int)l;
+        cnt[j] = (int)(r - l);
+        maxt = std::max(maxt, cnt[j]);
+    }
+    std::vector<float> wts((size_t)out * maxt, 0.0f);
+    for (int j = 0; j < out; j++) {
+        float c = ((float)j + 0.5f) * ratio - 0.5f;
+        float sum = 0.0f;
+        float* wr = &wts[(size_t)j * maxt];
+        for (int i = 0; i < cnt[j]; i++) {
+            float wv = lanczos3_f(((float)(left[j] + i) - c) / sratio);
+            wr[i] = wv;
+            sum += wv;
+        }
+        for (int i = 0; i < cnt[j]; i++) wr[i] /= sum;
+    }
+    LanczosRef R;
+    R.maxtaps = maxt;
+    R.left_off = tables.append(left.data(), left.size() * sizeof(int));
+    R.count_off = tables.append(cnt.data(), cnt.size() * sizeof(int));
+    R.w_off = tables.append(wts.data(), wts.size() * sizeof(float));
+    return lanczos.emplace(key, R).first->second;
+}
+
+// ---------------------------------------------------------------------------
+// Arena: one device allocation, carved per call; grows on demand.
+// ---------------------------------------------------------------------------
+int Context::ensure_arena(size_t bytes) {
+    if (bytes <= arena_cap) return SIFT_OK;
+    size_t ncap = std::max(bytes + bytes / 4, (size_t)64 << 20);
+    if (arena) {
+        HIP_TRY(hipStreamSynchronize(stream));
+        (void)hipFree(arena);
+        arena = nullptr;
+        arena_cap = 0;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, ncap);
+    if (e != hipSuccess) return fail(SIFT_ERR_NOMEM, std::string("device allocation failed: ") + hipGetErrorString(e));
+    arena = (char*)p;
+    arena_cap = ncap;
+    return SIFT_OK;
+}
+
+int Context::ensure_pinned(size_t bytes) {
+    if (bytes <= pinned_cap) return SIFT_OK;
+    size_t ncap = std::max(bytes + bytes / 4, (size_t)1 << 20);
+    if (pinned) {
+        HIP_TRY(hipStreamSynchronize(stream));
+        (void)hipHostFree(pinned);
+        pinned = nullptr;
+        pinned_cap = 0;
+    }
+    void* p = nullptr;
+    hipError_t e = hipHostMalloc(&p, ncap, hipHostMallocDefault);
+    if (e != hipSuccess) return fail(SIFT_ERR_NOMEM, "pinned host allocation failed");
+    pinned = (char*)p;
+    pinned_cap = ncap;
+    return SIFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Planning and launch
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Carver {
+    size_t off = 0;
+    size_t take(size_t bytes, size_t align = 256) {
+        off = round_up(off, align);
+        size_t r = off;
+        off += bytes;
+        return r;
+    }
+};
+
+struct LaunchRec {
+    int kind;  // 0 blur_h, 1 blur_v, 2 copy, 3 h_generic, 4 resize_v, 5 resize_h
+    size_t job_off;  // offset in the job blob
+    int njobs, tiles, max_pad;
+};
+
+}  // namespace
+
+int Context::run(const Batch& B, RunOutputs* outs) {
+    const int n = (int)B.images.size();
+    if (n == 0) return fail(SIFT_ERR_INVALID, "empty batch");
+    // --- tables (host) ---
+    std::vector<const ParamTables*> ptab(n);
+    for (int i = 0; i < n; i++) ptab[i] = &tables_for(B.params[i]);
+    struct OctShape { int w, h, pitch; };
+    std::vector<std::vector<OctShape>> shapes(n);
+    for (int i = 0; i < n; i++) {
+        int w = (int)B.images[i].width, h = (int)B.images[i].height;
+        for (uint32_t o = 0; o < B.params[i].num_octaves; o++) {
+            shapes[i].push_back({w, h, (int)round_up((size_t)std::max(w, 1), 64)});
+            w /= 2;
+            h /= 2;
+        }
+        for (uint32_t o = 0; o + 1 < B.params[i].num_octaves; o++) {
+            const OctShape& a = shapes[i][o];
+            const OctShape& b = shapes[i][o + 1];
+            if (a.w > 0 && a.h > 0 && b.w > 0 && b.h > 0) {
+                lanczos_for(a.h, b.h);
+                lanczos_for(a.w, b.w);
+            }
+        }
+    }
+    int rc = tables.ensure_uploaded();
+    if (rc) return rc;
+
+    // --- device layout ---
+    Carver C;
+    std::vector<size_t> in_off(n), htemp_off(n), rtmp_off(n);
+    std::vector<std::vector<size_t>> lev_off(n), dog_off(n);
+    std::vector<int> in_pitch(n);
+    for (int i = 0; i < n; i++) {
+        const sift_image_t& im = B.images[i];
+        int S = (int)B.params[i].num_intervals, L = S + 3;
+        if (B.device_input) {
+            in_pitch[i] = (int)(im.stride ? im.stride : im.width);
+        } else {
+            in_pitch[i] = (int)round_up(im.width, 64);
+            in_off[i] = C.take((size_t)in_pitch[i] * im.height);
+        }
+        lev_off[i].resize(shapes[i].size());
+        dog_off[i].resize(shapes[i].size());
+        for (size_t o = 0; o < shapes[i].size(); o++) {
+            const OctShape& s = shapes[i][o];
+            lev_off[i][o] = C.take((size_t)L * s.pitch * s.h + 64);
+            if (B.want_dog) dog_off[i][o] = C.take((size_t)(L - 1) * s.w * s.h * sizeof(float));
+        }
+        const OctShape& s0 = shapes[i][0];
+        htemp_off[i] = C.take((size_t)s0.pitch * s0.h + 64);
+        rtmp_off[i] = C.take((size_t)s0.pitch * std::max(s0.h / 2, 1) * sizeof(float));
+    }
+
+    // octave table for detection
+    std::vector<OctaveDesc> octs;
+    std::vector<int> img_slot_begin(n + 1);
+    int nslots = 0, total_rows = 0;
+    long long nbits = 0;
+    size_t pix_layers = 0;
+    for (int i = 0; i < n; i++) {
+        img_slot_begin[i] = nslots;
+        int S = (int)B.params[i].num_intervals, L = S + 3;
+        for (size_t o = 0; o < shapes[i].size(); o++) {
+            const OctShape& s = shapes[i][o];
+            if (s.w <= 0 || s.h <= 0) continue;
+            OctaveDesc d{};
+            d.level_stride = (long long)s.pitch * s.h;
+            d.w = s.w;
+            d.h = s.h;
+            d.pitch = s.pitch;
+            d.octave = (int)o;
+            d.img = i;
+            d.nlevels = L;
+            d.slot_begin = nslots;
+            d.bits_begin = nbits;
+            d.nwords = (s.w + 63) / 64;
+            d.row_begin = total_rows;
+            nslots += S * s.h;
+            nbits += (long long)S * s.h * d.nwords;
+            total_rows += s.h;
+            pix_layers += (size_t)S * s.w * s.h;
+            octs.push_back(d);
+        }
+    }
+    img_slot_begin[n] = nslots;
+    const int noct = (int)octs.size();
+
+    // capacities (grow-and-retry on overflow)
+    size_t want_cap = std::max<size_t>(4096, pix_layers / 48);
+    if (cand_cap < want_cap) cand_cap = want_cap;
+    if (kp_cap < cand_cap + cand_cap / 2) kp_cap = cand_cap + cand_cap / 2;
+    const int cap = (int)cand_cap, kcap = (int)kp_cap;
+
+    const size_t bits_off = C.take((size_t)std::max<long long>(nbits, 1) * 8);
+    const size_t counts_off = C.take((size_t)(nslots + 1) * 4);
+    const size_t offs_off = C.take((size_t)(nslots + 2) * 4);
+    const size_t scantmp1_off = C.take((size_t)(nslots / kScanTileElems + 4) * 4);
+    const size_t cand_off = C.take((size_t)cap * sizeof(Cand));
+    const size_t candresp_off = C.take((size_t)cap * 4);
+    const size_t ref_off = C.take((size_t)cap * sizeof(Keypoint));
+    const size_t valid_off = C.take((size_t)cap * 4);
+    const size_t npk_off = C.take((size_t)(cap + 1) * 4);
+    const size_t ang_off = C.take((size_t)cap * kMaxPeaks * 4);
+    const size_t koff_off = C.take((size_t)(cap + 2) * 4);
+    const size_t scantmp2_off = C.take((size_t)(cap / kScanTileElems + 4) * 4);
+    const size_t kps_off = C.take((size_t)kcap * sizeof(Keypoint));
+    const size_t desc_off = C.take((size_t)kcap * 128 * 4);
+    const size_t summary_off = C.take((size_t)(n + 3) * 4);
+
+    // --- job blob (host), uploaded to the end of the arena ---
+    std::vector<char> blob;
+    auto blob_take = [&](size_t bytes) {
+        size_t off = round_up(blob.size(), 256);
+        blob.resize(off + bytes);
+        return off;
+    };
+    std::vector<LaunchRec> launches;
+    int maxO = 0, Lmax = 0;
+    for (int i = 0; i < n; i++) {
+        maxO = std::max(maxO, (int)shapes[i].size());
+        Lmax = std::max(Lmax, (int)B.params[i].num_intervals + 3);
+    }
+    const size_t max_launches = (size_t)(1 + maxO * Lmax) * 4 + (size_t)maxO * 2 + 8;
+    const size_t max_jobs = (size_t)n * ((1 + (size_t)maxO * Lmax) * 2 + (size_t)maxO * 2);
+    const size_t blob_bound = max_launches * 256 + max_jobs * std::max(sizeof(BlurJob), sizeof(ResizeJob)) +
+                              octs.size() * sizeof(OctaveDesc) + (size_t)n * sizeof(ImageParams) +
+                              (size_t)(n + 1) * sizeof(int) + 4096;
+    const size_t blob_dev_off = round_up(C.off, 256);
+    rc = ensure_arena(blob_dev_off + blob_bound);
+    if (rc) return rc;
+    char* A = arena;
+    char* T = tables.dev;
+    auto AP = [&](size_t off) { return A + off; };
+
+    auto tiles_blur_h = [](int w, int h) { return ((w + 511) / 512) * ((h + 3) / 4); };
+
+    auto push_launch = [&](int kind, auto& jobs, int max_pad) {
+        if (jobs.empty()) return;
+        int tiles = 0;
+        for (auto& j : jobs) {
+            j.tile_begin = tiles;
+            tiles += j.tiles;
+        }
+        if (tiles == 0) return;
+        size_t off = blob_take(jobs.size() * sizeof(jobs[0]));
+        memcpy(blob.data() + off, jobs.data(), jobs.size() * sizeof(jobs[0]));
+        launches.push_back({kind, off, (int)jobs.size(), tiles, max_pad});
+    };
+
+    auto make_blur_pair = [&](const uint8_t* src, int src_pitch, uint8_t* tmp, int tmp_pitch,
+                              uint8_t* dst, int dst_pitch, int w, int h, const TapsRef& R,
+                              float* dog, const uint8_t* prev, int dog_pitch,
+                              std::vector<BlurJob>& hj, std::vector<BlurJob>& hgj,
+                              std::vector<BlurJob>& vj, std::vector<BlurJob>& cj) {
+        if (w <= 0 || h <= 0) return;
+        BlurJob J{};
+        J.w = w;
+        J.h = h;
+        if (R.copy) {
+            J.src = src;
+            J.src_pitch = src_pitch;
+            J.dst = dst;
+            J.dst_pitch = dst_pitch;
+            J.dog = dog;
+            J.dog_pitch = dog_pitch;
+            J.tiles_x = (w + 255) / 256;
+            J.tiles = J.tiles_x * h;
+            cj.push_back(J);
+            return;
+        }
+        J.taps = (const float*)(T + R.off);
+        J.ntaps = R.ntaps;
+        J.ntaps_pad = R.npad;
+        J.eps_rel = R.eps_rel;
+        BlurJob H = J;
+        H.src = src;
+        H.src_pitch = src_pitch;
+        H.dst = tmp;
+        H.dst_pitch = tmp_pitch;
+        if (R.npad <= kMaxHPad) {
+            H.tiles_x = (w + 511) / 512;
+            H.tiles = tiles_blur_h(w, h);
+            hj.push_back(H);
+        } else {
+            H.tiles_x = (w + 255) / 256;
+            H.tiles = H.tiles_x * h;
+            hgj.push_back(H);
+        }
+        BlurJob V = J;
+        V.src = tmp;
+        V.src_pitch = tmp_pitch;
+        V.dst = dst;
+        V.dst_pitch = dst_pitch;
+        V.dog = dog;
+        V.prev = prev;
+        V.dog_pitch = dog_pitch;
+        V.tiles_x = (w + 255) / 256;
+        V.tiles = V.tiles_x * ((h + 31) / 32);
+        vj.push_back(V);
+    };
+
+    auto flush_blur = [&](std::vector<BlurJob>& hj, std::vector<BlurJob>& hgj,
+                          std::vector<BlurJob>& vj, std::vector<BlurJob>& cj) {
+        int maxpad = 0;
+        for (auto& j : hj) maxpad = std::max(maxpad, j.ntaps_pad);
+        push_launch(0, hj, maxpad);
+        push_launch(3, hgj, 0);
+        push_launch(1, vj, 0);
+        push_launch(2, cj, 0);
+        hj.clear(); hgj.clear(); vj.clear(); cj.clear();
+    };
+
+    std::vector<BlurJob> hj, hgj, vj, cj;
+    // initial blur into octave 0 level 0
+    for (int i = 0; i < n; i++) {
+        const OctShape& s = shapes[i][0];
+        const uint8_t* src = B.device_input ? B.images[i].data : (const uint8_t*)AP(in_off[i]);
+        make_blur_pair(src, in_pitch[i], (uint8_t*)AP(htemp_off[i]), s.pitch,
+                       (uint8_t*)AP(lev_off[i][0]), s.pitch, s.w, s.h, ptab[i]->initial, nullptr,
+                       nullptr, 0, hj, hgj, vj, cj);
+    }
+    flush_blur(hj, hgj, vj, cj);
+    for (int o = 0; o < maxO; o++) {
+        for (int s = 1; s < Lmax; s++) {
+            for (int i = 0; i < n; i++) {
+                if ((int)shapes[i].size() <= o) continue;
+                const int L = (int)B.params[i].num_intervals + 3;
+                if (s >= L) continue;
+                const OctShape& sh = shapes[i][o];
+                const size_t lsz = (size_t)sh.pitch * sh.h;
+                uint8_t* base = (uint8_t*)AP(lev_off[i][o]);
+                float* dog = B.want_dog ? (float*)AP(dog_off[i][o]) + (size_t)(s - 1) * sh.w * sh.h : nullptr;
+                make_blur_pair(base + (s - 1) * lsz, sh.pitch, (uint8_t*)AP(htemp_off[i]), sh.pitch,
+                               base + s * lsz, sh.pitch, sh.w, sh.h, ptab[i]->level[(size_t)o * L + s],
+                               dog, base + (s - 1) * lsz, sh.w, hj, hgj, vj, cj);
+            }
+            flush_blur(hj, hgj, vj, cj);
+        }
+        // Lanczos3 to the next octave base (src/sift.rs:166-173)
+        std::vector<ResizeJob> rv, rh;
+        for (int i = 0; i < n; i++) {
+            if ((int)shapes[i].size() <= o + 1) continue;
+            const OctShape& a = shapes[i][o];
+            const OctShape& b = shapes[i][o + 1];
+            if (a.w <= 0 || a.h <= 0 || b.w <= 0 || b.h <= 0) continue;
+            const int S = (int)B.params[i].num_intervals;
+            const LanczosRef& LV = lanczos_for(a.h, b.h);
+            const LanczosRef& LH = lanczos_for(a.w, b.w);
+            ResizeJob J{};
+            J.src_u8 = (uint8_t*)AP(lev_off[i][o]) + (size_t)S * a.pitch * a.h;
+            J.tmp = (float*)AP(rtmp_off[i]);
+            J.dst = (uint8_t*)AP(lev_off[i][o + 1]);
+            J.in_w = a.w;
+            J.in_h = a.h;
+            J.out_w = b.w;
+            J.out_h = b.h;
+            J.src_pitch = a.pitch;
+            J.tmp_pitch = a.pitch;
+            J.dst_pitch = b.pitch;
+            ResizeJob V = J;
+            V.left = (const int*)(T + LV.left_off);
+            V.count = (const int*)(T + LV.count_off);
+            V.weights = (const float*)(T + LV.w_off);
+            V.maxtaps = LV.maxtaps;
+            V.tiles_x = (a.w + 255) / 256;
+            V.tiles = V.tiles_x * b.h;
+            rv.push_back(V);
+            ResizeJob H = J;
+            H.left = (const int*)(T + LH.left_off);
+            H.count = (const int*)(T + LH.count_off);
+            H.weights = (const float*)(T + LH.w_off);
+            H.maxtaps = LH.maxtaps;
+            H.tiles_x = (b.w + 255) / 256;
+            H.tiles = H.tiles_x * b.h;
+            rh.push_back(H);
+        }
+        push_launch(4, rv, 0);
+        push_launch(5, rh, 0);
+    }
+
+    // octave descriptors and per-image params
+    for (auto& d : octs) d.levels = (const uint8_t*)AP(lev_off[d.img][d.octave]);
+    std::vector<ImageParams> iparams(n);
+    for (int i = 0; i < n; i++) {
+        const sift_params_t& p = B.params[i];
+        iparams[i] = {p.sigma, p.contrast_threshold, p.edge_threshold, (int)p.num_intervals,
+                      (int)p.image_border_width, (int)p.num_octaves};
+    }
+    const size_t oct_blob = blob_take(std::max<size_t>(octs.size(), 1) * sizeof(OctaveDesc));
+    if (!octs.empty()) memcpy(blob.data() + oct_blob, octs.data(), octs.size() * sizeof(OctaveDesc));
+    const size_t par_blob = blob_take(n * sizeof(ImageParams));
+    memcpy(blob.data() + par_blob, iparams.data(), n * sizeof(ImageParams));
+    const size_t isb_blob = blob_take((n + 1) * sizeof(int));
+    memcpy(blob.data() + isb_blob, img_slot_begin.data(), (n + 1) * sizeof(int));
+
+    if (blob.size() > blob_bound) return fail(SIFT_ERR_DEVICE, "internal: job blob bound exceeded");
+
+    // staging: inputs + blob in pinned memory
+    size_t in_bytes = 0;
+    if (!B.device_input)
+        for (int i = 0; i < n; i++) in_bytes += (size_t)B.images[i].width * B.images[i].height;
+    rc = ensure_pinned(round_up(blob.size(), 256) + in_bytes + 256);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(stream));  // pinned staging is reused across calls
+    memcpy(pinned, blob.data(), blob.size());
+    hipStream_t st = stream;
+    HIP_TRY(hipMemcpyAsync(A + blob_dev_off, pinned, blob.size(), hipMemcpyHostToDevice, st));
+    if (!B.device_input) {
+        size_t po = round_up(blob.size(), 256);
+        for (int i = 0; i < n; i++) {
+            const sift_image_t& im = B.images[i];
+            const size_t stride = im.stride ? im.stride : im.width;
+            for (uint32_t y = 0; y < im.height; y++)
+                memcpy(pinned + po + (size_t)y * im.width, im.data + (size_t)y * stride, im.width);
+            HIP_TRY(hipMemcpy2DAsync(A + in_off[i], in_pitch[i], pinned + po, im.width, im.width,
+                                     im.height, hipMemcpyHostToDevice, st));
+            po += (size_t)im.width * im.height;
+        }
+    }
+    char* D = A + blob_dev_off;
+
+    // --- pyramid ---
+    for (const LaunchRec& L : launches) {
+        switch (L.kind) {
+            case 0: launch_blur_h((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.max_pad, st); break;
+            case 1: launch_blur_v((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
+            case 2: launch_copy_level((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
+            case 3: launch_blur_h_generic((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
+            case 4: launch_resize_v((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
+            case 5: launch_resize_h((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
+        }
+    }
+    HIP_TRY(hipGetLastError());
+
+    const OctaveDesc* d_oct = (const OctaveDesc*)(D + oct_blob);
+    const ImageParams* d_par = (const ImageParams*)(D + par_blob);
+    unsigned* d_counts = (unsigned*)AP(counts_off);
+    unsigned* d_offs = (unsigned*)AP(offs_off);
+    Cand* d_cand = (Cand*)AP(cand_off);
+    Keypoint* d_ref = (Keypoint*)AP(ref_off);
+    int* d_valid = (int*)AP(valid_off);
+    unsigned* d_npk = (unsigned*)AP(npk_off);
+    float* d_ang = (float*)AP(ang_off);
+    unsigned* d_koff = (unsigned*)AP(koff_off);
+    Keypoint* d_kps = (Keypoint*)AP(kps_off);
+    float* d_desc = (float*)AP(desc_off);
+    unsigned* d_sum = (unsigned*)AP(summary_off);
+    unsigned* d_total = d_offs + nslots;
+
+    if (!B.pyramid_only) {
+        // --- detection ---
+        launch_extrema(d_oct, noct, total_rows, d_par, (unsigned long long*)AP(bits_off), d_counts, st);
+        launch_scan_dev(d_counts, nslots, nullptr, d_offs, (unsigned*)AP(scantmp1_off), st);
+        launch_compact(d_oct, noct, nslots, (const unsigned long long*)AP(bits_off), d_counts, d_offs,
+                       d_cand, cap, st);
+        launch_refine(d_oct, d_par, d_cand, d_total, cap, d_ref, d_valid,
+                      B.want_stages ? (float*)AP(candresp_off) : nullptr, st);
+        launch_orient(d_oct, d_par, d_cand, d_total, cap, d_ref, d_valid, d_npk, d_ang, st);
+        launch_scan_dev(d_npk, cap, d_total, d_koff, (unsigned*)AP(scantmp2_off), st);
+        launch_descriptor(d_oct, d_par, d_cand, d_total, cap, d_ref, d_npk, d_ang, d_koff, kcap, d_kps,
+                          d_desc, st);
+        launch_finalize(d_offs, (const int*)(D + isb_blob), n, d_koff, d_total, cap, d_sum, st);
+        HIP_TRY(hipGetLastError());
+    }
+
+    if (outs) {
+        outs->d_kps = d_kps;
+        outs->d_desc = d_desc;
+        outs->d_summary = d_sum;
+        outs->d_cand = d_cand;
+        outs->d_cand_resp = (float*)AP(candresp_off);
+        outs->d_ref = d_ref;
+        outs->d_valid = d_valid;
+        outs->d_octs = d_oct;
+        outs->octs = octs;
+        outs->cap = cap;
+        outs->kcap = kcap;
+        outs->lev_off = lev_off;
+        outs->dog_off = dog_off;
+        outs->arena = A;
+    }
+    return SIFT_OK;
+}
+
+}  // namespace sh
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+namespace {
+
+int check_params(const sift_params_t* p) {
+    if (!p) return fail(SIFT_ERR_INVALID, "params is null");
+    if (p->num_intervals < 1 || p->num_intervals > 13) return fail(SIFT_ERR_INVALID, "num_intervals must be in [1, 13]");
+    if (p->num_octaves < 1 || p->num_octaves > 16) return fail(SIFT_ERR_INVALID, "num_octaves must be in [1, 16]");
+    if (!(p->sigma > 0.0f)) return fail(SIFT_ERR_INVALID, "sigma must be > 0");
+    return SIFT_OK;
+}
+
+int check_image(const sift_image_t* im) {
+    if (!im || !im->data) return fail(SIFT_ERR_INVALID, "image or image data is null");
+    if (im->width == 0 || im->height == 0) return fail(SIFT_ERR_INVALID, "image has zero size");
+    if (im->width > 65536 || im->height > 65536) return fail(SIFT_ERR_INVALID, "image too large");
+    if (im->stride && im->stride < im->width) return fail(SIFT_ERR_INVALID, "stride < width");
+    return SIFT_OK;
+}
+
+int collect(sh::Context* C, int n, sh::RunOutputs& R, sift_result_t* out, bool* retry) {
+    std::vector<unsigned> sum(n + 3);
+    HIP_TRY(hipMemcpyAsync(sum.data(), R.d_summary, (n + 3) * 4, hipMemcpyDeviceToHost, C->stream));
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    const size_t ncand = sum[0], nkp = sum[1];
+    *retry = false;
+    if (ncand > (size_t)R.cap || nkp > (size_t)R.kcap) {
+        C->cand_cap = std::max(C->cand_cap, ncand + ncand / 4 + 1024);
+        C->kp_cap = std::max(C->kp_cap, std::max(C->cand_cap + C->cand_cap / 2, nkp + nkp / 4 + 1024));
+        *retry = true;
+        return SIFT_OK;
+    }
+    if (!out) return SIFT_OK;
+    std::vector<sh::Keypoint> kps(nkp);
+    std::vector<float> desc(nkp * 128);
+    if (nkp) {
+        HIP_TRY(hipMemcpyAsync(kps.data(), R.d_kps, nkp * sizeof(sh::Keypoint), hipMemcpyDeviceToHost, C->stream));
+        HIP_TRY(hipMemcpyAsync(desc.data(), R.d_desc, nkp * 128 * 4, hipMemcpyDeviceToHost, C->stream));
+        HIP_TRY(hipStreamSynchronize(C->stream));
+    }
+    for (int i = 0; i < n; i++) {
+        size_t b = sum[2 + i], e = sum[3 + i];
+        size_t c = e - b;
+        out[i].count = c;
+        out[i].keypoints = nullptr;
+        out[i].descriptors = nullptr;
+        if (c) {
+            out[i].keypoints = (sift_keypoint_t*)malloc(c * sizeof(sift_keypoint_t));
+            out[i].descriptors = (float*)malloc(c * 128 * sizeof(float));
+            if (!out[i].keypoints || !out[i].descriptors) return fail(SIFT_ERR_NOMEM, "host allocation failed");
+            memcpy(out[i].keypoints, kps.data() + b, c * sizeof(sift_keypoint_t));
+            memcpy(out[i].descriptors, desc.data() + b * 128, c * 128 * sizeof(float));
+        }
+    }
+    return SIFT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void sift_params_default(sift_params_t* out) {
+    if (!out) return;
+    sift_params_new(out, 1.6f, 4, 3, 0.5f, 0.04f, 10.0f);
+}
+
+void sift_params_new(sift_params_t* out, float sigma, uint32_t num_octaves, uint32_t num_intervals,
+                     float assumed_blur, float contrast_threshold, float edge_threshold) {
+    if (!out) return;
+    out->sigma = sigma;
+    out->num_octaves = num_octaves;
+    out->num_intervals = num_intervals;
+    out->assumed_blur = assumed_blur;
+    out->contrast_threshold = contrast_threshold / (float)num_intervals;
+    out->edge_threshold = edge_threshold;
+    out->image_border_width = 5;
+}
+
+const char* sift_hip_last_error(void) { return g_last_error.c_str(); }
+
+int sift_hip_create(const sift_params_t* params, int device, sift_hip_ctx** out) {
+    if (!out) return fail(SIFT_ERR_INVALID, "out is null");
+    *out = nullptr;
+    int rc = check_params(params);
+    if (rc) return rc;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) return fail(SIFT_ERR_DEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(SIFT_ERR_DEVICE, "device ordinal out of range");
+    HIP_TRY(hipSetDevice(device));
+    sh::Context* C = new (std::nothrow) sh::Context();
+    if (!C) return fail(SIFT_ERR_NOMEM, "context allocation failed");
+    C->device = device;
+    C->params = *params;
+    e = hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete C;
+        return fail(SIFT_ERR_DEVICE, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+    }
+    *out = reinterpret_cast<sift_hip_ctx*>(C);
+    return SIFT_OK;
+}
+
+void sift_hip_destroy(sift_hip_ctx* ctx) {
+    if (!ctx) return;
+    sh::Context* C = reinterpret_cast<sh::Context*>(ctx);
+    sift_hip_server_stop(ctx);
+    (void)hipSetDevice(C->device);
+    if (C->stream) (void)hipStreamSynchronize(C->stream);
+    if (C->arena) (void)hipFree(C->arena);
+    if (C->tables.dev) (void)hipFree(C->tables.dev);
+    if (C->pinned) (void)hipHostFree(C->pinned);
+    if (C->stream) (void)hipStreamDestroy(C->stream);
+    delete C;
+}
+
+int sift_hip_detect_batch(sift_hip_ctx* ctx, const sift_image_t* images, const sift_params_t* params,
+                          size_t n, sift_result_t* out) {
+    if (!ctx || !images || !out || n == 0) return fail(SIFT_ERR_INVALID, "null argument or empty batch");
+    sh::Context* C = reinterpret_cast<sh::Context*>(ctx);
+    for (size_t i = 0; i < n; i++) {
+        out[i] = sift_result_t{nullptr, nullptr, 0};
+        int rc = check_image(&images[i]);
+        if (rc) return rc;
+        if (params && (rc = check_params(&params[i]))) return rc;
+    }
+    std::lock_guard<std::mutex> lk(C->mu);
+    (void)hipSetDevice(C->device);
+    sh::Batch B;
+    B.images.assign(images, images + n);
+    B.params.assign(n, C->params);
+    if (params) B.params.assign(params, params + n);
+    for (int attempt = 0; attempt < 4; attempt++) {
+        sh::RunOutputs R;
+        int rc = C->run(B, &R);
+        if (rc) return rc;
+        bool retry = false;
+        rc = collect(C, (int)n, R, out, &retry);
+        if (rc) return rc;
+        if (!retry) return SIFT_OK;
+    }
+    return fail(SIFT_ERR_NOMEM, "keypoint capacity did not converge");
+}
+
+int sift_hip_detect(sift_hip_ctx* ctx, const sift_image_t* image, sift_result_t* out) {
+    return sift_hip_detect_batch(ctx, image, nullptr, 1, out);
+}
+
+int sift_hip_detect_batch_device(sift_hip_ctx* ctx, const sift_image_t* images,
+                                 const sift_params_t* params, size_t n, size_t* out_counts,
+                                 const sift_keypoint_t** d_keypoints, const float** d_descriptors,
+                                 const uint32_t** d_offsets) {
+    if (!ctx || !images || n == 0) return fail(SIFT_ERR_INVALID, "null argument or empty batch");
+    sh::Context* C = reinterpret_cast<sh::Context*>(ctx);
+    for (size_t i = 0; i < n; i++) {
+        int rc = check_image(&images[i]);
+        if (rc) return rc;
+        if (params && (rc = check_params(&params[i]))) return rc;
+    }
+    std::lock_guard<std::mutex> lk(C->mu);
+    (void)hipSetDevice(C->device);
+    sh::Batch B;
+    B.images.assign(images, images + n);
+    B.params.assign(n, C->params);
+    if (params) B.params.assign(params, params + n);
+    B.device_input = true;
+    for (int attempt = 0; attempt < 4; attempt++) {
+        sh::RunOutputs R;
+        int rc = C->run(B, &R);
+        if (rc) return rc;
+        std::vector<unsigned> sum(n + 3);
+        HIP_TRY(hipMemcpyAsync(sum.data(), R.d_summary, (n + 3) * 4, hipMemcpyDeviceToHost, C->stream));
+        HIP_TRY(hipStreamSynchronize(C->stream));
+        if (sum[0] > (unsigned)R.cap || sum[1] > (unsigned)R.kcap) {
+            C->cand_cap = std::max(C->cand_cap, (size_t)sum[0] + sum[0] / 4 + 1024);
+            C->kp_cap = std::max(C->kp_cap, std::max(C->cand_cap + C->cand_cap / 2, (size_t)sum[1] + sum[1] / 4 + 1024));
+            continue;
+        }
+        if (out_counts)
+            for (size_t i = 0; i < n; i++) out_counts[i] = sum[3 + i] - sum[2 + i];
+        if (d_keypoints) *d_keypoints = (const sift_keypoint_t*)R.d_kps;
+        if (d_descriptors) *d_descriptors = R.d_desc;
+        if (d_offsets) *d_offsets = R.d_summary + 2;
+        return SIFT_OK;
+    }
+    return fail(SIFT_ERR_NOMEM, "keypoint capacity did not converge");
+}
+
+void sift_result_free(sift_result_t* r) {
+    if (!r) return;
+    free(r->keypoints);
+    free(r->descriptors);
+    r->keypoints = nullptr;
+    r->descriptors = nullptr;
+    r->count = 0;
+}
+
+int sift_hip_pyramid(sift_hip_ctx* ctx, const sift_image_t* image, uint8_t* gauss, float* dog) {
+    if (!ctx || !gauss) return fail(SIFT_ERR_INVALID, "null argument");
+    int rc = check_image(image);
+    if (rc) return rc;
+    sh::Context* C = reinterpret_cast<sh::Context*>(ctx);
+    std::lock_guard<std::mutex> lk(C->mu);
+    (void)hipSetDevice(C->device);
+    sh::Batch B;
+    B.images.assign(image, image + 1);
+    B.params.assign(1, C->params);
+    B.pyramid_only = true;
+    B.want_dog = dog != nullptr;
+    sh::RunOutputs R;
+    rc = C->run(B, &R);
+    if (rc) return rc;
+    const int L = (int)C->params.num_intervals + 3;
+    int w = (int)image->width, h = (int)image->height;
+    size_t go = 0, dgo = 0;
+    for (uint32_t o = 0; o < C->params.num_octaves; o++) {
+        const int pitch = (int)round_up((size_t)std::max(w, 1), 64);
+        if (w > 0 && h > 0) {
+            for (int s = 0; s < L; s++) {
+                HIP_TRY(hipMemcpy2DAsync(gauss + go + (size_t)s * w * h, w,
+                                         R.arena + R.lev_off[0][o] + (size_t)s * pitch * h, pitch, w, h,
+                                         hipMemcpyDeviceToHost, C->stream));
+            }
+            if (dog)
+                HIP_TRY(hipMemcpyAsync(dog + dgo, R.arena + R.dog_off[0][o], (size_t)(L - 1) * w * h * 4,
+                                       hipMemcpyDeviceToHost, C->stream));
+        }
+        go += (size_t)L * w * h;
+        dgo += (size_t)(L - 1) * w * h;
+        w /= 2;
+        h /= 2;
+    }
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    return SIFT_OK;
+}
+
+int sift_hip_detect_stages(sift_hip_ctx* ctx, const sift_image_t* image, sift_stages_t* out) {
+    if (!ctx || !out) return fail(SIFT_ERR_INVALID, "null argument");
+    memset(out, 0, sizeof(*out));
+    int rc = check_image(image);
+    if (rc) return rc;
+    sh::Context* C = reinterpret_cast<sh::Context*>(ctx);
+    std::lock_guard<std::mutex> lk(C->mu);
+    (void)hipSetDevice(C->device);
+    sh::Batch B;
+    B.images.assign(image, image + 1);
+    B.params.assign(1, C->params);
+    B.want_stages = true;
+    for (int attempt = 0; attempt < 4; attempt++) {
+        sh::RunOutputs R;
+        rc = C->run(B, &R);
+        if (rc) return rc;
+        bool retry = false;
+        rc = collect(C, 1, R, &out->result, &retry);
+        if (rc) return rc;
+        if (retry) continue;
+        unsigned nc = 0;
+        HIP_TRY(hipMemcpy(&nc, R.d_summary, 4, hipMemcpyDeviceToHost));
+        std::vector<sh::Cand> cand(nc);
+        std::vector<float> resp(nc);
+        std::vector<sh::Keypoint> ref(nc);
+        std::vector<int> valid(nc);
+        if (nc) {
+            HIP_TRY(hipMemcpy(cand.data(), R.d_cand, nc * sizeof(sh::Cand), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(resp.data(), R.d_cand_resp, nc * 4, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(ref.data(), R.d_ref, nc * sizeof(sh::Keypoint), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(valid.data(), R.d_valid, nc * 4, hipMemcpyDeviceToHost));
+        }
+        out->candidates = (sift_keypoint_t*)malloc(std::max<size_t>(nc, 1) * sizeof(sift_keypoint_t));
+        out->n_candidates = nc;
+        size_t nr = 0;
+        for (unsigned c = 0; c < nc; c++) nr += valid[c] ? 1 : 0;
+        out->refined = (sift_keypoint_t*)malloc(std::max<size_t>(nr, 1) * sizeof(sift_keypoint_t));
+        out->n_refined = nr;
+        size_t k = 0;
+        for (unsigned c = 0; c < nc; c++) {
+            const sh::OctaveDesc& O = R.octs[cand[c].job];
+            const float scale = ldexpf(1.0f, O.octave);
+            sift_keypoint_t& kp = out->candidates[c];
+            kp.x = ((float)cand[c].x + 0.5f) * scale;
+            kp.y = ((float)cand[c].y + 0.5f) * scale;
+            kp.size = 0.0f;
+            kp.angle = 0.0f;
+            kp.response = resp[c];
+            kp.octave = O.octave;
+            kp.layer = cand[c].layer;
+            if (valid[c]) memcpy(&out->refined[k++], &ref[c], sizeof(sift_keypoint_t));
+        }
+        return SIFT_OK;
+    }
+    return fail(SIFT_ERR_NOMEM, "keypoint capacity did not converge");
+}
+
+void sift_stages_free(sift_stages_t* s) {
+    if (!s) return;
+    free(s->candidates);
+    free(s->refined);
+    sift_result_free(&s->result);
+    s->candidates = nullptr;
+    s->refined = nullptr;
+    s->n_candidates = s->n_refined = 0;
+}
+
+}  // extern "C"

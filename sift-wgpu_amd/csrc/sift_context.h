@@ -1,0 +1,110 @@
+// sift_context.h -- host-side context of libsift_hip.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "sift_hip.h"
+#include "sift_internal.h"
+
+namespace sh {
+
+struct TapsRef {
+    int ntaps = 0, npad = 0;
+    float eps_rel = 0.0f;
+    size_t off = 0;      // byte offset in the table store
+    bool copy = false;   // blur sigma < 1e-4: the level is a copy of the previous one
+};
+
+struct ParamTables {
+    TapsRef initial;
+    std::vector<TapsRef> level;  // [o * (S+3) + s], s >= 1
+};
+
+struct LanczosRef {
+    int maxtaps = 0;
+    size_t left_off = 0, count_off = 0, w_off = 0;
+};
+
+struct ParamKey {
+    float sigma, assumed_blur;
+    uint32_t S, O;
+    bool operator<(const ParamKey& o) const {
+        return std::tie(sigma, assumed_blur, S, O) < std::tie(o.sigma, o.assumed_blur, o.S, o.O);
+    }
+};
+
+// Host mirror + device copy of all weight tables, append-only.
+struct TableStore {
+    std::vector<char> host;
+    char* dev = nullptr;
+    size_t dev_cap = 0, uploaded = 0;
+    size_t append(const void* p, size_t bytes);
+    int ensure_uploaded();
+};
+
+struct Batch {
+    std::vector<sift_image_t> images;
+    std::vector<sift_params_t> params;
+    bool device_input = false;
+    bool pyramid_only = false;
+    bool want_dog = false;
+    bool want_stages = false;
+};
+
+struct RunOutputs {
+    Keypoint* d_kps = nullptr;
+    float* d_desc = nullptr;
+    unsigned* d_summary = nullptr;
+    Cand* d_cand = nullptr;
+    float* d_cand_resp = nullptr;
+    Keypoint* d_ref = nullptr;
+    int* d_valid = nullptr;
+    const OctaveDesc* d_octs = nullptr;
+    std::vector<OctaveDesc> octs;
+    int cap = 0, kcap = 0;
+    std::vector<std::vector<size_t>> lev_off, dog_off;
+    char* arena = nullptr;
+};
+
+struct Server;
+
+struct Context {
+    int device = 0;
+    sift_params_t params{};
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+
+    TableStore tables;
+    std::map<ParamKey, ParamTables> param_tables;
+    std::map<std::pair<int, int>, LanczosRef> lanczos;
+
+    char* arena = nullptr;
+    size_t arena_cap = 0;
+    char* pinned = nullptr;
+    size_t pinned_cap = 0;
+    size_t cand_cap = 0, kp_cap = 0;
+
+    Server* server = nullptr;  // owned; created by sift_hip_server_start
+
+    const ParamTables& tables_for(const sift_params_t& p);
+    const LanczosRef& lanczos_for(int in, int out);
+    int ensure_arena(size_t bytes);
+    int ensure_pinned(size_t bytes);
+    // Enqueues the whole pipeline for a batch on `stream` (no host sync except
+    // for table uploads); outputs stay in the arena until the next run.
+    int run(const Batch& B, RunOutputs* outs);
+};
+
+}  // namespace sh

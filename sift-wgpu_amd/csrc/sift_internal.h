@@ -1,0 +1,123 @@
+// sift_internal.h -- device-visible job descriptors and host launch wrappers
+// shared by the gfx950 kernels and the host orchestration of libsift_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sh {
+
+// Gaussian taps are zero-padded to a multiple of this (register tap block).
+constexpr int kTapBlock = 8;
+// Max orientation peaks per keypoint: strict local maxima of a circular 36-bin
+// histogram (src/sift.rs:691-719) are at most 18.
+constexpr int kMaxPeaks = 18;
+// Fixed-point scale of the histogram accumulators (int64 LDS atomics): sums are
+// order-independent, hence deterministic.
+constexpr double kFixScale = 1099511627776.0;  // 2^40
+
+// One separable-blur pass over one image level (horizontal or vertical).
+struct BlurJob {
+    const uint8_t* src;
+    uint8_t* dst;
+    const uint8_t* prev;  // V pass DoG epilogue: level s-1 (may be null)
+    float* dog;           // V pass DoG epilogue: dog = q[dst] - q[prev] (may be null)
+    const float* taps;    // ntaps_pad floats, zero-padded
+    int ntaps;            // 2r+1
+    int ntaps_pad;        // multiple of kTapBlock
+    float eps_rel;        // fast-path error bound factor for ntaps taps
+    int w, h;
+    int src_pitch, dst_pitch, dog_pitch;
+    int tiles_x;          // tiles along x for this job
+    int tile_begin;       // prefix over jobs in the launch
+    int tiles;            // tiles of this job
+};
+
+// One Lanczos3 resize pass (vertical: u8 -> f32 rows; horizontal: f32 -> u8).
+struct ResizeJob {
+    const uint8_t* src_u8;  // vertical input
+    float* tmp;             // vertical output / horizontal input (pitch tmp_pitch floats)
+    uint8_t* dst;           // horizontal output
+    const int* left;        // per output index
+    const int* count;
+    const float* weights;   // [out][maxtaps]
+    int maxtaps;
+    int in_w, in_h, out_w, out_h;
+    int src_pitch, tmp_pitch, dst_pitch;
+    int tiles_x;
+    int tile_begin;
+    int tiles;
+};
+
+// Per-image parameters read by the detection kernels.
+struct ImageParams {
+    float sigma;
+    float contrast_threshold;  // stored value (already / S)
+    float edge_threshold;
+    int num_intervals;
+    int border;
+    int num_octaves;
+};
+
+// One (image, octave) of the pyramid.
+struct OctaveDesc {
+    const uint8_t* levels;  // level s at levels + s*level_stride
+    long long level_stride;
+    int w, h, pitch;
+    int octave;
+    int img;
+    int nlevels;            // S+3
+    int slot_begin;         // first extremum row-slot of this octave: (s-1)*h + y
+    long long bits_begin;   // first u64 word of the extremum bitmask
+    int nwords;             // u64 words per row-slot
+    int row_begin;          // prefix of rows over octaves (extremum launch grid)
+};
+
+// Candidate extremum (before refinement): octave job index, position, layer.
+struct Cand {
+    int job;      // index into the OctaveDesc table
+    int x, y;
+    int layer;
+};
+
+struct Keypoint {
+    float x, y, size, angle, response;
+    int octave, layer;
+};
+
+// Launch wrappers (kernels_pyramid.hip / kernels_detect.hip).
+void launch_blur_h(const BlurJob* d_jobs, int njobs, int total_tiles, int max_pad, hipStream_t s);
+void launch_blur_v(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
+void launch_resize_v(const ResizeJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
+void launch_resize_h(const ResizeJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
+void launch_copy_level(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
+void launch_blur_h_generic(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
+void launch_blur_v_generic(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
+
+void launch_extrema(const OctaveDesc* d_oct, int noct, int total_rows, const ImageParams* d_params,
+                    unsigned long long* d_bits, unsigned* d_counts, hipStream_t s);
+void launch_scan_dev(const unsigned* d_in, int n_max, const unsigned* n_dev, unsigned* d_out,
+                     unsigned* d_tmp, hipStream_t s);
+void launch_compact(const OctaveDesc* d_oct, int noct, int nslots, const unsigned long long* d_bits,
+                    const unsigned* d_counts, const unsigned* d_offsets, Cand* d_cand, int cap,
+                    hipStream_t s);
+void launch_refine(const OctaveDesc* d_oct, const ImageParams* d_params, const Cand* d_cand,
+                   const unsigned* d_total, int cap, Keypoint* d_ref, int* d_valid,
+                   float* d_cand_resp, hipStream_t s);
+void launch_orient(const OctaveDesc* d_oct, const ImageParams* d_params, const Cand* d_cand,
+                   const unsigned* d_total, int cap, const Keypoint* d_ref, const int* d_valid,
+                   unsigned* d_npeaks, float* d_angles, hipStream_t s);
+void launch_descriptor(const OctaveDesc* d_oct, const ImageParams* d_params, const Cand* d_cand,
+                       const unsigned* d_total, int cap, const Keypoint* d_ref,
+                       const unsigned* d_npeaks, const float* d_angles, const unsigned* d_koff,
+                       int kcap, Keypoint* d_kps, float* d_desc, hipStream_t s);
+// Writes summary[0] = candidates, summary[1] = keypoints, summary[2..2+nimg] =
+// per-image keypoint offsets (nimg+1 entries).
+void launch_finalize(const unsigned* d_slot_offsets, const int* d_img_slot_begin, int nimg,
+                     const unsigned* d_koff, const unsigned* d_total, int cap,
+                     unsigned* d_summary, hipStream_t s);
+// Max padded tap count handled by the LDS row-tile horizontal pass.
+constexpr int kMaxHPad = 3576;
+constexpr int kScanTileElems = 4096;
+
+}  // namespace sh

@@ -1,0 +1,371 @@
+"""Python mirror of the reference's SIFT API over the C ABI of libsift_hip.so.
+
+Mirrors (stevel705/sift-wgpu):
+  Sift, Sift.new, Sift.default        src/sift.rs:36-80
+  Sift.detect_and_compute_with_backend src/sift.rs:1244-1355
+  SiftBackend + from_str aliases       src/lib.rs:18-49 (plus "hip")
+  KeyPoint                             src/keypoints.rs:1-10
+
+Only the Hip backend is implemented here; the reference's Cpu / WebGpu arms
+live in the Rust crate and are not reachable from this library, so asking for
+them raises SiftError (no silent CPU fallback). The library must be built
+(__graft_entry__.build()); importing works without a GPU, but creating a
+context on a machine without a HIP device raises SiftError.
+"""
+import ctypes
+import enum
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsift_hip.so")
+
+KEYPOINT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("size", "<f4"), ("angle", "<f4"),
+                           ("response", "<f4"), ("octave", "<i4"), ("layer", "<i4")])
+
+SIFT_OK = 0
+
+
+class SiftError(RuntimeError):
+    pass
+
+
+class SiftParams(ctypes.Structure):
+    _fields_ = [("sigma", ctypes.c_float), ("num_octaves", ctypes.c_uint32),
+                ("num_intervals", ctypes.c_uint32), ("assumed_blur", ctypes.c_float),
+                ("contrast_threshold", ctypes.c_float), ("edge_threshold", ctypes.c_float),
+                ("image_border_width", ctypes.c_uint32)]
+
+
+class SiftImage(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32), ("stride", ctypes.c_uint32)]
+
+
+class SiftResult(ctypes.Structure):
+    _fields_ = [("keypoints", ctypes.c_void_p), ("descriptors", ctypes.POINTER(ctypes.c_float)),
+                ("count", ctypes.c_size_t)]
+
+
+class SiftStages(ctypes.Structure):
+    _fields_ = [("candidates", ctypes.c_void_p), ("n_candidates", ctypes.c_size_t),
+                ("refined", ctypes.c_void_p), ("n_refined", ctypes.c_size_t),
+                ("result", SiftResult)]
+
+
+class SiftServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_batch", ctypes.c_uint32),
+                ("max_queue", ctypes.c_uint32)]
+
+
+class SiftCompletion(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64), ("status", ctypes.c_int), ("result", SiftResult),
+                ("submit_ms", ctypes.c_double), ("start_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double)]
+
+
+# Every symbol include/sift_hip.h declares (checked by the CPU test suite).
+EXPORTED_SYMBOLS = [
+    "sift_params_default", "sift_params_new", "sift_hip_create", "sift_hip_destroy",
+    "sift_hip_last_error", "sift_hip_detect", "sift_hip_detect_batch",
+    "sift_hip_detect_batch_device", "sift_result_free", "sift_hip_pyramid",
+    "sift_hip_detect_stages", "sift_stages_free", "sift_hip_server_start", "sift_hip_submit",
+    "sift_hip_poll", "sift_hip_server_stop",
+]
+
+_LIB = None
+_LIB_LOCK = threading.Lock()
+
+
+def lib():
+    """Loads libsift_hip.so; raises SiftError if it is not built."""
+    global _LIB
+    with _LIB_LOCK:
+        if _LIB is not None:
+            return _LIB
+        if not os.path.exists(LIB_PATH):
+            raise SiftError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        L.sift_params_default.argtypes = [P(SiftParams)]
+        L.sift_params_new.argtypes = [P(SiftParams), ctypes.c_float, ctypes.c_uint32,
+                                      ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float]
+        L.sift_hip_create.argtypes = [P(SiftParams), ctypes.c_int, P(ctypes.c_void_p)]
+        L.sift_hip_create.restype = ctypes.c_int
+        L.sift_hip_destroy.argtypes = [ctypes.c_void_p]
+        L.sift_hip_last_error.restype = ctypes.c_char_p
+        L.sift_hip_detect.argtypes = [ctypes.c_void_p, P(SiftImage), P(SiftResult)]
+        L.sift_hip_detect.restype = ctypes.c_int
+        L.sift_hip_detect_batch.argtypes = [ctypes.c_void_p, P(SiftImage), P(SiftParams),
+                                            ctypes.c_size_t, P(SiftResult)]
+        L.sift_hip_detect_batch.restype = ctypes.c_int
+        L.sift_hip_detect_batch_device.argtypes = [
+            ctypes.c_void_p, P(SiftImage), P(SiftParams), ctypes.c_size_t, P(ctypes.c_size_t),
+            P(ctypes.c_void_p), P(ctypes.c_void_p), P(ctypes.c_void_p)]
+        L.sift_hip_detect_batch_device.restype = ctypes.c_int
+        L.sift_result_free.argtypes = [P(SiftResult)]
+        L.sift_hip_pyramid.argtypes = [ctypes.c_void_p, P(SiftImage), P(ctypes.c_uint8),
+                                       P(ctypes.c_float)]
+        L.sift_hip_pyramid.restype = ctypes.c_int
+        L.sift_hip_detect_stages.argtypes = [ctypes.c_void_p, P(SiftImage), P(SiftStages)]
+        L.sift_hip_detect_stages.restype = ctypes.c_int
+        L.sift_stages_free.argtypes = [P(SiftStages)]
+        L.sift_hip_server_start.argtypes = [ctypes.c_void_p, P(SiftServerConfig)]
+        L.sift_hip_server_start.restype = ctypes.c_int
+        L.sift_hip_submit.argtypes = [ctypes.c_void_p, P(SiftImage), ctypes.c_uint64]
+        L.sift_hip_submit.restype = ctypes.c_int
+        L.sift_hip_poll.argtypes = [ctypes.c_void_p, P(SiftCompletion), ctypes.c_size_t,
+                                    P(ctypes.c_size_t), ctypes.c_int]
+        L.sift_hip_poll.restype = ctypes.c_int
+        L.sift_hip_server_stop.argtypes = [ctypes.c_void_p]
+        L.sift_hip_server_stop.restype = ctypes.c_int
+        _LIB = L
+        return L
+
+
+def last_error():
+    return lib().sift_hip_last_error().decode(errors="replace")
+
+
+def _check(rc, what):
+    if rc != SIFT_OK:
+        raise SiftError(f"{what} failed ({rc}): {last_error()}")
+
+
+class SiftBackend(enum.Enum):
+    """src/lib.rs:18-28 plus the Hip arm this library provides."""
+    Cpu = "cpu"
+    WebGpu = "webgpu"
+    WebGpuWithCpuFallback = "auto"
+    WebGpuV2 = "gpuv2"
+    Hip = "hip"
+
+    @staticmethod
+    def from_str(value):
+        """FromStr aliases of src/lib.rs:36-49, plus "hip" / "rocm"."""
+        v = value.strip().lower()
+        table = {"cpu": SiftBackend.Cpu,
+                 "webgpu": SiftBackend.WebGpu, "wgpu": SiftBackend.WebGpu, "gpu": SiftBackend.WebGpu,
+                 "auto": SiftBackend.WebGpuWithCpuFallback,
+                 "fallback": SiftBackend.WebGpuWithCpuFallback,
+                 "prefer-gpu": SiftBackend.WebGpuWithCpuFallback,
+                 "gpuv2": SiftBackend.WebGpuV2, "gpu-v2": SiftBackend.WebGpuV2,
+                 "webgpu-v2": SiftBackend.WebGpuV2, "v2": SiftBackend.WebGpuV2,
+                 "hip": SiftBackend.Hip, "rocm": SiftBackend.Hip}
+        if v not in table:
+            raise ValueError(f"Unknown SIFT backend: {value}")
+        return table[v]
+
+
+def params_struct(sigma, num_octaves, num_intervals, assumed_blur, contrast_threshold,
+                  edge_threshold):
+    p = SiftParams()
+    lib().sift_params_new(ctypes.byref(p), sigma, num_octaves, num_intervals, assumed_blur,
+                          contrast_threshold, edge_threshold)
+    return p
+
+
+class _Ctx:
+    """Process-global context per device (the Rust arm keeps one in a OnceLock)."""
+    _lock = threading.Lock()
+    _ctx = {}
+
+    @classmethod
+    def get(cls, device=0):
+        with cls._lock:
+            if device not in cls._ctx:
+                p = SiftParams()
+                lib().sift_params_default(ctypes.byref(p))
+                h = ctypes.c_void_p()
+                _check(lib().sift_hip_create(ctypes.byref(p), device, ctypes.byref(h)),
+                       "sift_hip_create")
+                cls._ctx[device] = h
+            return cls._ctx[device]
+
+
+def _image_struct(img):
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    if img.ndim != 2:
+        raise ValueError("expected a 2-D u8 grey image")
+    h, w = img.shape
+    return img, SiftImage(img.ctypes.data, w, h, w)
+
+
+def _take_result(r):
+    n = int(r.count)
+    if n:
+        kb = (ctypes.c_char * (n * KEYPOINT_DTYPE.itemsize)).from_address(r.keypoints)
+        kps = np.frombuffer(bytes(kb), KEYPOINT_DTYPE).copy()
+        desc = np.ctypeslib.as_array(r.descriptors, shape=(n * 128,)).copy().reshape(n, 128)
+    else:
+        kps = np.zeros(0, KEYPOINT_DTYPE)
+        desc = np.zeros((0, 128), np.float32)
+    lib().sift_result_free(ctypes.byref(r))
+    return kps, desc
+
+
+def _take_kps(ptr, n):
+    if n == 0 or not ptr:
+        return np.zeros(0, KEYPOINT_DTYPE)
+    kb = (ctypes.c_char * (n * KEYPOINT_DTYPE.itemsize)).from_address(ptr)
+    return np.frombuffer(bytes(kb), KEYPOINT_DTYPE).copy()
+
+
+class Sift:
+    """`Sift` of src/sift.rs:36-80. contrast_threshold is the stored value
+    (divided by num_intervals in new(), as the reference does)."""
+
+    def __init__(self, sigma=1.6, num_octaves=4, num_intervals=3, assumed_blur=0.5,
+                 contrast_threshold=0.04, edge_threshold=10.0, device=0):
+        self.sigma = float(sigma)
+        self.num_octaves = int(num_octaves)
+        self.num_intervals = int(num_intervals)
+        self.assumed_blur = float(assumed_blur)
+        self.contrast_threshold = float(np.float32(contrast_threshold) / np.float32(num_intervals))
+        self.edge_threshold = float(edge_threshold)
+        self.image_border_width = 5
+        self.device = device
+
+    @classmethod
+    def new(cls, sigma, num_octaves, num_intervals, assumed_blur, contrast_threshold,
+            edge_threshold):
+        return cls(sigma, num_octaves, num_intervals, assumed_blur, contrast_threshold,
+                   edge_threshold)
+
+    @classmethod
+    def default(cls):
+        return cls()
+
+    def params(self):
+        p = SiftParams()
+        p.sigma = self.sigma
+        p.num_octaves = self.num_octaves
+        p.num_intervals = self.num_intervals
+        p.assumed_blur = self.assumed_blur
+        p.contrast_threshold = self.contrast_threshold
+        p.edge_threshold = self.edge_threshold
+        p.image_border_width = self.image_border_width
+        return p
+
+    # -- the reference's entry points ---------------------------------------
+    def detect_and_compute_with_backend(self, img, backend):
+        if isinstance(backend, str):
+            backend = SiftBackend.from_str(backend)
+        if backend is not SiftBackend.Hip:
+            raise SiftError(f"backend {backend.name} is not provided by libsift_hip "
+                            "(only SiftBackend.Hip is)")
+        return self.detect_batch([img])[0]
+
+    def detect_and_compute(self, img):
+        return self.detect_and_compute_with_backend(img, SiftBackend.Hip)
+
+    # -- batched / device / introspection ------------------------------------
+    def detect_batch(self, images, params=None):
+        ctx = _Ctx.get(self.device)
+        keep = []
+        ims = (SiftImage * len(images))()
+        for i, im in enumerate(images):
+            a, s = _image_struct(im)
+            keep.append(a)
+            ims[i] = s
+        pa = None
+        if params is not None:
+            pa = (SiftParams * len(images))()
+            for i, p in enumerate(params):
+                pa[i] = p.params() if isinstance(p, Sift) else p
+        else:
+            pa = (SiftParams * len(images))()
+            for i in range(len(images)):
+                pa[i] = self.params()
+        outs = (SiftResult * len(images))()
+        _check(lib().sift_hip_detect_batch(ctx, ims, pa, len(images), outs),
+               "sift_hip_detect_batch")
+        return [_take_result(outs[i]) for i in range(len(images))]
+
+    def detect_batch_device(self, images, params=None):
+        """images: list of (device_ptr, w, h, stride). Returns (counts, kp_ptr,
+        desc_ptr, offsets_ptr); results stay on the device."""
+        ctx = _Ctx.get(self.device)
+        n = len(images)
+        ims = (SiftImage * n)()
+        for i, (ptr, w, h, stride) in enumerate(images):
+            ims[i] = SiftImage(ptr, w, h, stride)
+        pa = (SiftParams * n)()
+        for i in range(n):
+            p = params[i] if params is not None else self
+            pa[i] = p.params() if isinstance(p, Sift) else p
+        counts = (ctypes.c_size_t * n)()
+        kp = ctypes.c_void_p()
+        de = ctypes.c_void_p()
+        of = ctypes.c_void_p()
+        _check(lib().sift_hip_detect_batch_device(ctx, ims, pa, n, counts, ctypes.byref(kp),
+                                                  ctypes.byref(de), ctypes.byref(of)),
+               "sift_hip_detect_batch_device")
+        return list(counts), kp.value, de.value, of.value
+
+    def pyramid(self, img):
+        """Gaussian (u8) and DoG (f32) pyramids: lists over octaves of arrays
+        (S+3, H, W) and (S+2, H, W)."""
+        ctx = _Ctx.get(self.device)
+        a, s = _image_struct(img)
+        h, w = a.shape
+        L = self.num_intervals + 3
+        shapes = []
+        ww, hh = w, h
+        for _ in range(self.num_octaves):
+            shapes.append((ww, hh))
+            ww //= 2
+            hh //= 2
+        gsz = sum(L * x * y for x, y in shapes)
+        dsz = sum((L - 1) * x * y for x, y in shapes)
+        g = np.zeros(max(gsz, 1), np.uint8)
+        d = np.zeros(max(dsz, 1), np.float32)
+        # the context's params are used for the pyramid: set them per call
+        hh_ = ctypes.c_void_p()
+        p = self.params()
+        _check(lib().sift_hip_create(ctypes.byref(p), self.device, ctypes.byref(hh_)),
+               "sift_hip_create")
+        try:
+            _check(lib().sift_hip_pyramid(hh_, ctypes.byref(s),
+                                          g.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                          d.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                   "sift_hip_pyramid")
+        finally:
+            lib().sift_hip_destroy(hh_)
+        gs, ds = [], []
+        go = do = 0
+        for (x, y) in shapes:
+            gs.append(g[go:go + L * x * y].reshape(L, y, x))
+            ds.append(d[do:do + (L - 1) * x * y].reshape(L - 1, y, x))
+            go += L * x * y
+            do += (L - 1) * x * y
+        return gs, ds
+
+    def detect_stages(self, img):
+        """(candidates, refined, keypoints, descriptors) of one image."""
+        a, s = _image_struct(img)
+        h_ = ctypes.c_void_p()
+        p = self.params()
+        _check(lib().sift_hip_create(ctypes.byref(p), self.device, ctypes.byref(h_)),
+               "sift_hip_create")
+        st = SiftStages()
+        try:
+            _check(lib().sift_hip_detect_stages(h_, ctypes.byref(s), ctypes.byref(st)),
+                   "sift_hip_detect_stages")
+            cand = _take_kps(st.candidates, int(st.n_candidates))
+            ref = _take_kps(st.refined, int(st.n_refined))
+            n = int(st.result.count)
+            if n:
+                kps = _take_kps(st.result.keypoints, n)
+                desc = np.ctypeslib.as_array(st.result.descriptors, shape=(n * 128,)).copy()
+                desc = desc.reshape(n, 128)
+            else:
+                kps = np.zeros(0, KEYPOINT_DTYPE)
+                desc = np.zeros((0, 128), np.float32)
+            lib().sift_stages_free(ctypes.byref(st))
+        finally:
+            lib().sift_hip_destroy(h_)
+        return cand, ref, kps, desc

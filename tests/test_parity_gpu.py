@@ -1,0 +1,150 @@
+"""HIP path vs the CPU oracle, through the C ABI (libsift_hip.so).
+
+Bars: u8 pyramid levels and f32 DoG levels bit-exact (<= 1e-5 max-abs on the
+[0,1] scale is the stated tolerance); extremum candidates and refined keypoints
+bit-exact (pure + - * / arithmetic in the reference order); final keypoints
+matched within 0.05 px / 0.05 size / 1 degree with precision and recall
+>= 99.5%; matched descriptors within 1e-3 L2 (orientation and descriptor use
+the device's atan2f/expf/sinf/cosf, which may differ from glibc's by an ulp).
+"""
+import numpy as np
+import pytest
+
+import imgen
+import oracle
+import sift_hip
+from parity import DESC_L2_TOL, MIN_PR, check_keypoint_properties, match_keypoints
+
+pytestmark = pytest.mark.gpu
+
+
+def _sift(octaves=4, contrast=0.04, S=3, edge=10.0, sigma=1.6, blur=0.5):
+    return (sift_hip.Sift(sigma, octaves, S, blur, contrast, edge),
+            oracle.params(sigma, octaves, S, blur, contrast, edge))
+
+
+CASES = {
+    "c1": lambda: imgen.config_image("c1"),
+    "c2": lambda: imgen.config_image("c2"),
+    "odd_97x61": lambda: (imgen.generic_image(97, 61, seed=9), 3, 0.04),
+    "wide_333x45": lambda: (imgen.generic_image(333, 45, seed=10), 4, 0.04),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_pyramid_and_dog_bit_exact(case):
+    img, octv, ct = CASES[case]()
+    s, p = _sift(octv, ct)
+    g_hip, d_hip = s.pyramid(img)
+    g_ref, d_ref = oracle.build_pyramid(p, img)
+    for o in range(octv):
+        assert np.array_equal(g_hip[o], g_ref[o]), f"gauss octave {o}"
+        assert np.array_equal(d_hip[o], d_ref[o]), f"dog octave {o}"
+
+
+@pytest.mark.parametrize("case", ["c1", "c2", "odd_97x61"])
+def test_candidates_and_refined_bit_exact(case):
+    img, octv, ct = CASES[case]()
+    s, p = _sift(octv, ct)
+    cand, ref, kps, desc = s.detect_stages(img)
+    _, _, ocand, oref, _ = oracle.detect_and_compute(p, img, stages=True)
+    assert len(cand) == len(ocand)
+    assert cand.tobytes() == ocand.tobytes()
+    assert len(ref) == len(oref)
+    for f in ("x", "y", "response", "octave", "layer"):
+        assert np.array_equal(ref[f], oref[f]), f
+    # size = sigma * 2^(o + l/S): exp2 via a correctly rounded double path vs glibc
+    assert np.allclose(ref["size"], oref["size"], rtol=2e-7, atol=0)
+
+
+def _parity(img, s, p):
+    kps, desc = s.detect_and_compute(img)
+    okps, odesc = oracle.detect_and_compute(p, img)
+    m = match_keypoints(okps, kps, odesc, desc)
+    return kps, desc, okps, m
+
+
+@pytest.mark.parametrize("case", ["c1", "c2", "odd_97x61", "wide_333x45"])
+def test_keypoints_and_descriptors_match_oracle(case):
+    img, octv, ct = CASES[case]()
+    s, p = _sift(octv, ct)
+    kps, desc, okps, m = _parity(img, s, p)
+    print(f"{case}: oracle {m['n_ref']} hip {m['n_got']} matched {m['matched']} "
+          f"P {m['precision']:.5f} R {m['recall']:.5f} desc_l2_max {m['desc_l2_max']:.2e}")
+    assert m["precision"] >= MIN_PR and m["recall"] >= MIN_PR
+    assert m["desc_l2_max"] <= DESC_L2_TOL
+    check_keypoint_properties(kps, img.shape[1], img.shape[0])
+
+
+@pytest.mark.parametrize("variant", [
+    dict(octaves=5, contrast=0.03, S=3, edge=10.0),
+    dict(octaves=4, contrast=0.04, S=2, edge=8.0),
+    dict(octaves=3, contrast=0.02, S=4, edge=12.0, sigma=1.4, blur=0.6),
+])
+def test_parameter_variants(variant):
+    img = imgen.generic_image(300, 220, seed=31)
+    s, p = _sift(**variant)
+    g_hip, _ = s.pyramid(img)
+    g_ref, _ = oracle.build_pyramid(p, img)
+    for a, b in zip(g_hip, g_ref):
+        assert np.array_equal(a, b)
+    _, _, _, m = _parity(img, s, p)
+    assert m["precision"] >= MIN_PR and m["recall"] >= MIN_PR
+    assert m["desc_l2_max"] <= DESC_L2_TOL
+
+
+def test_uniform_image_has_no_keypoints():
+    # tests/integration_test.rs:205-226
+    s, _ = _sift()
+    kps, desc = s.detect_and_compute(np.full((100, 100), 128, np.uint8))
+    assert len(kps) == 0 and len(desc) == 0
+
+
+@pytest.mark.parametrize("shape", [(20, 15), (1, 1), (3, 2), (17, 9), (64, 1), (1, 64), (11, 11)])
+def test_tiny_and_degenerate_images(shape):
+    # tests/integration_test.rs:166-203 (20x15 gradient) and smaller: octaves
+    # down to 0 pixels must not fault and must match the oracle exactly.
+    w, h = shape
+    img = np.tile((np.arange(w) * 10 % 256).astype(np.uint8), (h, 1))
+    s, p = _sift(4)
+    kps, desc = s.detect_and_compute(img)
+    okps, odesc = oracle.detect_and_compute(p, img)
+    assert len(kps) == len(desc) == len(okps)
+    check_keypoint_properties(kps, w, h)
+    g_hip, _ = s.pyramid(img)
+    g_ref, _ = oracle.build_pyramid(p, img)
+    for a, b in zip(g_hip, g_ref):
+        assert np.array_equal(a, b)
+
+
+def test_determinism():
+    # tests/integration_test.rs:106-164: bitwise-equal repeated runs
+    img, octv, ct = imgen.config_image("c1")
+    s, _ = _sift(octv, ct)
+    k1, d1 = s.detect_and_compute(img)
+    k2, d2 = s.detect_and_compute(img)
+    assert k1.tobytes() == k2.tobytes() and d1.tobytes() == d2.tobytes()
+
+
+def test_batch_equals_alone():
+    """Each image's result is the same alone and inside a mixed-size batch."""
+    shapes = [(640, 480), (97, 61), (1280, 720), (333, 45), (640, 480)]
+    imgs = [imgen.generic_image(w, h, seed=100 + i) for i, (w, h) in enumerate(shapes)]
+    params = [sift_hip.Sift(1.6, imgen.octaves_for(w, h) if min(w, h) >= 64 else 3, 3, 0.5, 0.04, 10.0)
+              for (w, h) in shapes]
+    s = sift_hip.Sift.default()
+    together = s.detect_batch(imgs, params)
+    for i, img in enumerate(imgs):
+        alone = params[i].detect_and_compute(img)
+        assert alone[0].tobytes() == together[i][0].tobytes(), i
+        assert alone[1].tobytes() == together[i][1].tobytes(), i
+
+
+def test_high_contrast_threshold():
+    # tests/integration_test.rs:228-281
+    img, octv, _ = imgen.config_image("c1")
+    kd, _ = sift_hip.Sift(1.6, octv, 3, 0.5, 0.04, 10.0).detect_and_compute(img)
+    kh, dh = sift_hip.Sift(1.6, octv, 3, 0.5, 1.0, 10.0).detect_and_compute(img)
+    assert len(kd) > 0
+    assert len(kh) < len(kd) // 2 or len(kh) == 0
+    assert len(kh) == len(dh)
