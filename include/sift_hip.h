@@ -137,6 +137,22 @@ typedef struct sift_stages_t {
 int sift_hip_detect_stages(sift_hip_ctx* ctx, const sift_image_t* image, sift_stages_t* out);
 void sift_stages_free(sift_stages_t* s);
 
+/* ---- Kernel timing --------------------------------------------------------
+ * When enabled, every kernel launch of the context is bracketed by HIP events
+ * on the context's stream; durations and the launch's algorithmic HBM bytes
+ * (u8 levels in and out for the blur and resize passes, levels read plus bitmask
+ * written for the extremum scan; 0 for the gather-bound keypoint stages) are
+ * accumulated per kernel name. Enabling resets the counters. */
+typedef struct sift_kernel_stat_t {
+    char name[32];
+    uint64_t launches;
+    double total_ms;
+    double bytes;
+} sift_kernel_stat_t;
+
+int sift_hip_profile(sift_hip_ctx* ctx, int enable);
+int sift_hip_profile_read(sift_hip_ctx* ctx, sift_kernel_stat_t* out, size_t cap, size_t* n);
+
 /* ---- Serving: continuous batching on one device -------------------------
  * Frames are admitted while device memory for their pyramids fits within
  * `hbm_budget_bytes`; frames arriving while others run are batched into the

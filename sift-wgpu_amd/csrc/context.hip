@@ -13,6 +13,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "sift_hip.h"
@@ -251,12 +252,78 @@ struct LaunchRec {
     int kind;  // 0 blur_h, 1 blur_v, 2 copy, 3 h_generic, 4 resize_v, 5 resize_h
     size_t job_off;  // offset in the job blob
     int njobs, tiles, max_pad;
+    double bytes;    // algorithmic HBM bytes of the launch
 };
+
+enum KernelId {
+    K_BLUR_H = 0, K_BLUR_V, K_COPY, K_BLUR_H_GENERIC, K_RESIZE_V, K_RESIZE_H, K_EXTREMA, K_SCAN,
+    K_COMPACT, K_REFINE, K_ORIENT, K_DESCRIPTOR, K_FINALIZE, K_COUNT
+};
+const char* const kKernelNames[K_COUNT] = {
+    "k_blur_h", "k_blur_v", "k_copy_level", "k_blur_h_generic", "k_resize_v", "k_resize_h",
+    "k_extrema", "k_scan", "k_compact", "k_refine", "k_orient", "k_descriptor", "k_finalize"};
+
+double blur_job_bytes(const BlurJob& j, int kind) {
+    double a = (double)j.w * j.h;
+    if (kind == K_BLUR_V && j.dog) return a * (2.0 + 1.0 + 4.0);
+    if (kind == K_COPY && j.dog) return a * (2.0 + 4.0);
+    return a * 2.0;
+}
+double resize_job_bytes(const ResizeJob& j, int kind) {
+    if (kind == K_RESIZE_V) return (double)j.in_w * j.in_h + (double)j.in_w * j.out_h * 4.0;
+    return (double)j.in_w * j.out_h * 4.0 + (double)j.out_w * j.out_h;
+}
 
 }  // namespace
 
+hipEvent_t Context::prof_event() {
+    if (!prof_events.empty()) {
+        hipEvent_t e = prof_events.back();
+        prof_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void Context::prof_begin(int kernel, double bytes, hipEvent_t* a) {
+    (void)kernel;
+    (void)bytes;
+    *a = nullptr;
+    if (!profiling) return;
+    *a = prof_event();
+    if (*a) (void)hipEventRecord(*a, stream);
+}
+
+void Context::prof_end(int kernel, double bytes, hipEvent_t a) {
+    if (!profiling || !a) return;
+    hipEvent_t b = prof_event();
+    if (!b) return;
+    (void)hipEventRecord(b, stream);
+    prof_pending.push_back({kernel, bytes, a, b});
+}
+
+void Context::prof_resolve() {
+    for (const ProfPending& p : prof_pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            ProfStat& S = prof_stats[p.kernel];
+            S.launches++;
+            S.total_ms += ms;
+            S.bytes += p.bytes;
+        }
+        prof_events.push_back(p.a);
+        prof_events.push_back(p.b);
+    }
+    prof_pending.clear();
+}
+
 int Context::run(const Batch& B, RunOutputs* outs) {
     const int n = (int)B.images.size();
+    // every public entry point synchronises before returning, so the previous
+    // call's events are complete
+    prof_resolve();
     if (n == 0) return fail(SIFT_ERR_INVALID, "empty batch");
     // --- tables (host) ---
     std::vector<const ParamTables*> ptab(n);
@@ -391,17 +458,24 @@ int Context::run(const Batch& B, RunOutputs* outs) {
 
     auto tiles_blur_h = [](int w, int h) { return ((w + 511) / 512) * ((h + 3) / 4); };
 
+    auto job_bytes = [](const auto& j, int kind) -> double {
+        using J = std::decay_t<decltype(j)>;
+        if constexpr (std::is_same_v<J, BlurJob>) return blur_job_bytes(j, kind);
+        else return resize_job_bytes(j, kind);
+    };
     auto push_launch = [&](int kind, auto& jobs, int max_pad) {
         if (jobs.empty()) return;
         int tiles = 0;
+        double bytes = 0.0;
         for (auto& j : jobs) {
             j.tile_begin = tiles;
             tiles += j.tiles;
+            bytes += job_bytes(j, kind);
         }
         if (tiles == 0) return;
         size_t off = blob_take(jobs.size() * sizeof(jobs[0]));
         memcpy(blob.data() + off, jobs.data(), jobs.size() * sizeof(jobs[0]));
-        launches.push_back({kind, off, (int)jobs.size(), tiles, max_pad});
+        launches.push_back({kind, off, (int)jobs.size(), tiles, max_pad, bytes});
     };
 
     auto make_blur_pair = [&](const uint8_t* src, int src_pitch, uint8_t* tmp, int tmp_pitch,
@@ -578,6 +652,8 @@ int Context::run(const Batch& B, RunOutputs* outs) {
 
     // --- pyramid ---
     for (const LaunchRec& L : launches) {
+        hipEvent_t ev = nullptr;
+        prof_begin(L.kind, L.bytes, &ev);
         switch (L.kind) {
             case 0: launch_blur_h((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.max_pad, st); break;
             case 1: launch_blur_v((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
@@ -586,6 +662,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             case 4: launch_resize_v((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
             case 5: launch_resize_h((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
         }
+        prof_end(L.kind, L.bytes, ev);
     }
     HIP_TRY(hipGetLastError());
 
@@ -606,17 +683,37 @@ int Context::run(const Batch& B, RunOutputs* outs) {
 
     if (!B.pyramid_only) {
         // --- detection ---
+        double ext_bytes = 0.0;
+        for (const auto& d : octs)
+            ext_bytes += (double)d.nlevels * d.w * d.h + (double)(d.nlevels - 3) * d.h * d.nwords * 8.0;
+        hipEvent_t ev = nullptr;
+        prof_begin(K_EXTREMA, ext_bytes, &ev);
         launch_extrema(d_oct, noct, total_rows, d_par, (unsigned long long*)AP(bits_off), d_counts, st);
+        prof_end(K_EXTREMA, ext_bytes, ev);
+        prof_begin(K_SCAN, 0.0, &ev);
         launch_scan_dev(d_counts, nslots, nullptr, d_offs, (unsigned*)AP(scantmp1_off), st);
+        prof_end(K_SCAN, 0.0, ev);
+        prof_begin(K_COMPACT, 0.0, &ev);
         launch_compact(d_oct, noct, nslots, (const unsigned long long*)AP(bits_off), d_counts, d_offs,
                        d_cand, cap, st);
+        prof_end(K_COMPACT, 0.0, ev);
+        prof_begin(K_REFINE, 0.0, &ev);
         launch_refine(d_oct, d_par, d_cand, d_total, cap, d_ref, d_valid,
                       B.want_stages ? (float*)AP(candresp_off) : nullptr, st);
+        prof_end(K_REFINE, 0.0, ev);
+        prof_begin(K_ORIENT, 0.0, &ev);
         launch_orient(d_oct, d_par, d_cand, d_total, cap, d_ref, d_valid, d_npk, d_ang, st);
+        prof_end(K_ORIENT, 0.0, ev);
+        prof_begin(K_SCAN, 0.0, &ev);
         launch_scan_dev(d_npk, cap, d_total, d_koff, (unsigned*)AP(scantmp2_off), st);
+        prof_end(K_SCAN, 0.0, ev);
+        prof_begin(K_DESCRIPTOR, 0.0, &ev);
         launch_descriptor(d_oct, d_par, d_cand, d_total, cap, d_ref, d_npk, d_ang, d_koff, kcap, d_kps,
                           d_desc, st);
+        prof_end(K_DESCRIPTOR, 0.0, ev);
+        prof_begin(K_FINALIZE, 0.0, &ev);
         launch_finalize(d_offs, (const int*)(D + isb_blob), n, d_koff, d_total, cap, d_sum, st);
+        prof_end(K_FINALIZE, 0.0, ev);
         HIP_TRY(hipGetLastError());
     }
 
@@ -751,6 +848,8 @@ void sift_hip_destroy(sift_hip_ctx* ctx) {
     sift_hip_server_stop(ctx);
     (void)hipSetDevice(C->device);
     if (C->stream) (void)hipStreamSynchronize(C->stream);
+    C->prof_resolve();
+    for (hipEvent_t e : C->prof_events) (void)hipEventDestroy(e);
     if (C->arena) (void)hipFree(C->arena);
     if (C->tables.dev) (void)hipFree(C->tables.dev);
     if (C->pinned) (void)hipHostFree(C->pinned);
@@ -933,6 +1032,40 @@ int sift_hip_detect_stages(sift_hip_ctx* ctx, const sift_image_t* image, sift_st
         return SIFT_OK;
     }
     return fail(SIFT_ERR_NOMEM, "keypoint capacity did not converge");
+}
+
+int sift_hip_profile(sift_hip_ctx* ctx, int enable) {
+    if (!ctx) return fail(SIFT_ERR_INVALID, "null context");
+    sh::Context* C = reinterpret_cast<sh::Context*>(ctx);
+    std::lock_guard<std::mutex> lk(C->mu);
+    (void)hipSetDevice(C->device);
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    C->prof_resolve();
+    C->prof_stats.clear();
+    C->profiling = enable != 0;
+    return SIFT_OK;
+}
+
+int sift_hip_profile_read(sift_hip_ctx* ctx, sift_kernel_stat_t* out, size_t cap, size_t* n) {
+    if (!ctx || !n) return fail(SIFT_ERR_INVALID, "null argument");
+    sh::Context* C = reinterpret_cast<sh::Context*>(ctx);
+    std::lock_guard<std::mutex> lk(C->mu);
+    (void)hipSetDevice(C->device);
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    C->prof_resolve();
+    size_t k = 0;
+    for (const auto& kv : C->prof_stats) {
+        if (out && k < cap) {
+            memset(&out[k], 0, sizeof(out[k]));
+            strncpy(out[k].name, kKernelNames[kv.first], sizeof(out[k].name) - 1);
+            out[k].launches = kv.second.launches;
+            out[k].total_ms = kv.second.total_ms;
+            out[k].bytes = kv.second.bytes;
+        }
+        k++;
+    }
+    *n = k;
+    return SIFT_OK;
 }
 
 void sift_stages_free(sift_stages_t* s) {

@@ -80,6 +80,17 @@ struct RunOutputs {
 
 struct Server;
 
+// Per-kernel event timing (sift_hip_profile).
+struct ProfPending {
+    int kernel;
+    double bytes;
+    hipEvent_t a, b;
+};
+struct ProfStat {
+    uint64_t launches = 0;
+    double total_ms = 0.0, bytes = 0.0;
+};
+
 struct Context {
     int device = 0;
     sift_params_t params{};
@@ -97,6 +108,15 @@ struct Context {
     size_t cand_cap = 0, kp_cap = 0;
 
     Server* server = nullptr;  // owned; created by sift_hip_server_start
+
+    bool profiling = false;
+    std::vector<ProfPending> prof_pending;
+    std::vector<hipEvent_t> prof_events;  // free pool
+    std::map<int, ProfStat> prof_stats;
+    hipEvent_t prof_event();
+    void prof_begin(int kernel, double bytes, hipEvent_t* a);
+    void prof_end(int kernel, double bytes, hipEvent_t a);
+    void prof_resolve();  // call after the stream is idle
 
     const ParamTables& tables_for(const sift_params_t& p);
     const LanczosRef& lanczos_for(int in, int out);

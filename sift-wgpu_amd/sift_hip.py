@@ -55,6 +55,11 @@ class SiftStages(ctypes.Structure):
                 ("result", SiftResult)]
 
 
+class SiftKernelStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_uint64),
+                ("total_ms", ctypes.c_double), ("bytes", ctypes.c_double)]
+
+
 class SiftServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_batch", ctypes.c_uint32),
                 ("max_queue", ctypes.c_uint32)]
@@ -71,8 +76,8 @@ EXPORTED_SYMBOLS = [
     "sift_params_default", "sift_params_new", "sift_hip_create", "sift_hip_destroy",
     "sift_hip_last_error", "sift_hip_detect", "sift_hip_detect_batch",
     "sift_hip_detect_batch_device", "sift_result_free", "sift_hip_pyramid",
-    "sift_hip_detect_stages", "sift_stages_free", "sift_hip_server_start", "sift_hip_submit",
-    "sift_hip_poll", "sift_hip_server_stop",
+    "sift_hip_detect_stages", "sift_stages_free", "sift_hip_profile", "sift_hip_profile_read",
+    "sift_hip_server_start", "sift_hip_submit", "sift_hip_poll", "sift_hip_server_stop",
 ]
 
 _LIB = None
@@ -113,6 +118,11 @@ def lib():
         L.sift_hip_detect_stages.argtypes = [ctypes.c_void_p, P(SiftImage), P(SiftStages)]
         L.sift_hip_detect_stages.restype = ctypes.c_int
         L.sift_stages_free.argtypes = [P(SiftStages)]
+        L.sift_hip_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.sift_hip_profile.restype = ctypes.c_int
+        L.sift_hip_profile_read.argtypes = [ctypes.c_void_p, P(SiftKernelStat), ctypes.c_size_t,
+                                            P(ctypes.c_size_t)]
+        L.sift_hip_profile_read.restype = ctypes.c_int
         L.sift_hip_server_start.argtypes = [ctypes.c_void_p, P(SiftServerConfig)]
         L.sift_hip_server_start.restype = ctypes.c_int
         L.sift_hip_submit.argtypes = [ctypes.c_void_p, P(SiftImage), ctypes.c_uint64]
@@ -369,3 +379,20 @@ class Sift:
         finally:
             lib().sift_hip_destroy(h_)
         return cand, ref, kps, desc
+
+
+def profile(enable=True, device=0):
+    """Enables (and resets) or disables per-kernel event timing on the
+    process-global context."""
+    _check(lib().sift_hip_profile(_Ctx.get(device), 1 if enable else 0), "sift_hip_profile")
+
+
+def profile_read(device=0):
+    """{kernel: (launches, total_ms, algorithmic_bytes)} since profile(True)."""
+    ctx = _Ctx.get(device)
+    n = ctypes.c_size_t()
+    _check(lib().sift_hip_profile_read(ctx, None, 0, ctypes.byref(n)), "sift_hip_profile_read")
+    arr = (SiftKernelStat * max(n.value, 1))()
+    _check(lib().sift_hip_profile_read(ctx, arr, n.value, ctypes.byref(n)), "sift_hip_profile_read")
+    return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms), float(arr[i].bytes))
+            for i in range(n.value)}
