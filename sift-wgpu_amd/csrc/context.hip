@@ -128,8 +128,6 @@ const ParamTables& Context::tables_for(const sift_params_t& p) {
         taps.resize(npad, 0.0f);
         out.ntaps = n;
         out.npad = npad;
-        // 2.05 (n+2) u bounds 2 g(n+1) / (1 - g(n)) with margin (u = 2^-24).
-        out.eps_rel = (float)(2.05 * (double)(n + 2) * 5.9604644775390625e-8);
         out.off = tables.append(taps.data(), taps.size() * sizeof(float));
         out.copy = false;
     };
@@ -249,23 +247,27 @@ struct Carver {
 };
 
 struct LaunchRec {
-    int kind;  // 0 blur_h, 1 blur_v, 2 copy, 3 h_generic, 4 resize_v, 5 resize_h
+    int kind;        // KernelId
     size_t job_off;  // offset in the job blob
     int njobs, tiles, max_pad;
+    int variant;     // k_blur_h: outputs per lane; k_blur_v_narrow: staged rows
     double bytes;    // algorithmic HBM bytes of the launch
 };
 
 enum KernelId {
     K_BLUR_H = 0, K_BLUR_V, K_COPY, K_BLUR_H_GENERIC, K_RESIZE_V, K_RESIZE_H, K_EXTREMA, K_SCAN,
-    K_COMPACT, K_REFINE, K_ORIENT, K_DESCRIPTOR, K_FINALIZE, K_COUNT
+    K_COMPACT, K_REFINE, K_ORIENT, K_DESCRIPTOR, K_FINALIZE, K_BLUR_V_NARROW, K_BLUR_V_GENERIC,
+    K_COUNT
 };
 const char* const kKernelNames[K_COUNT] = {
     "k_blur_h", "k_blur_v", "k_copy_level", "k_blur_h_generic", "k_resize_v", "k_resize_h",
-    "k_extrema", "k_scan", "k_compact", "k_refine", "k_orient", "k_descriptor", "k_finalize"};
+    "k_extrema", "k_scan", "k_compact", "k_refine", "k_orient", "k_descriptor", "k_finalize",
+    "k_blur_v_narrow", "k_blur_v_generic"};
 
 double blur_job_bytes(const BlurJob& j, int kind) {
     double a = (double)j.w * j.h;
-    if (kind == K_BLUR_V && j.dog) return a * (2.0 + 1.0 + 4.0);
+    if ((kind == K_BLUR_V || kind == K_BLUR_V_NARROW || kind == K_BLUR_V_GENERIC) && j.dog)
+        return a * (2.0 + 1.0 + 4.0);
     if (kind == K_COPY && j.dog) return a * (2.0 + 4.0);
     return a * 2.0;
 }
@@ -375,7 +377,11 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         rtmp_off[i] = C.take((size_t)s0.pitch * std::max(s0.h / 2, 1) * sizeof(float));
     }
 
-    // octave table for detection
+    // octave table for detection; the tiled extremum kernel needs one S per batch
+    int S0 = (int)B.params[0].num_intervals;
+    bool tile_ext = extrema_tile_supported(S0);
+    for (int i = 1; i < n; i++) tile_ext = tile_ext && (int)B.params[i].num_intervals == S0;
+    int total_tiles = 0;
     std::vector<OctaveDesc> octs;
     std::vector<int> img_slot_begin(n + 1);
     int nslots = 0, total_rows = 0;
@@ -398,7 +404,8 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             d.slot_begin = nslots;
             d.bits_begin = nbits;
             d.nwords = (s.w + 63) / 64;
-            d.row_begin = total_rows;
+            d.row_begin = tile_ext ? total_tiles : total_rows;
+            total_tiles += ((s.w + 63) / 64) * ((s.h + 15) / 16);
             nslots += S * s.h;
             nbits += (long long)S * s.h * d.nwords;
             total_rows += s.h;
@@ -444,7 +451,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         maxO = std::max(maxO, (int)shapes[i].size());
         Lmax = std::max(Lmax, (int)B.params[i].num_intervals + 3);
     }
-    const size_t max_launches = (size_t)(1 + maxO * Lmax) * 4 + (size_t)maxO * 2 + 8;
+    const size_t max_launches = (size_t)(1 + maxO * Lmax) * 8 + (size_t)maxO * 2 + 8;
     const size_t max_jobs = (size_t)n * ((1 + (size_t)maxO * Lmax) * 2 + (size_t)maxO * 2);
     const size_t blob_bound = max_launches * 256 + max_jobs * std::max(sizeof(BlurJob), sizeof(ResizeJob)) +
                               octs.size() * sizeof(OctaveDesc) + (size_t)n * sizeof(ImageParams) +
@@ -456,14 +463,15 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     char* T = tables.dev;
     auto AP = [&](size_t off) { return A + off; };
 
-    auto tiles_blur_h = [](int w, int h) { return ((w + 511) / 512) * ((h + 3) / 4); };
+    std::vector<BlurJob> hj, hj2, hgj, vj, vnj, vgj, cj;
+    int vn_rows = 0;
 
     auto job_bytes = [](const auto& j, int kind) -> double {
         using J = std::decay_t<decltype(j)>;
         if constexpr (std::is_same_v<J, BlurJob>) return blur_job_bytes(j, kind);
         else return resize_job_bytes(j, kind);
     };
-    auto push_launch = [&](int kind, auto& jobs, int max_pad) {
+    auto push_launch = [&](int kind, auto& jobs, int max_pad, int variant) {
         if (jobs.empty()) return;
         int tiles = 0;
         double bytes = 0.0;
@@ -475,14 +483,12 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         if (tiles == 0) return;
         size_t off = blob_take(jobs.size() * sizeof(jobs[0]));
         memcpy(blob.data() + off, jobs.data(), jobs.size() * sizeof(jobs[0]));
-        launches.push_back({kind, off, (int)jobs.size(), tiles, max_pad, bytes});
+        launches.push_back({kind, off, (int)jobs.size(), tiles, max_pad, variant, bytes});
     };
 
     auto make_blur_pair = [&](const uint8_t* src, int src_pitch, uint8_t* tmp, int tmp_pitch,
                               uint8_t* dst, int dst_pitch, int w, int h, const TapsRef& R,
-                              float* dog, const uint8_t* prev, int dog_pitch,
-                              std::vector<BlurJob>& hj, std::vector<BlurJob>& hgj,
-                              std::vector<BlurJob>& vj, std::vector<BlurJob>& cj) {
+                              float* dog, const uint8_t* prev, int dog_pitch) {
         if (w <= 0 || h <= 0) return;
         BlurJob J{};
         J.w = w;
@@ -502,16 +508,17 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         J.taps = (const float*)(T + R.off);
         J.ntaps = R.ntaps;
         J.ntaps_pad = R.npad;
-        J.eps_rel = R.eps_rel;
         BlurJob H = J;
         H.src = src;
         H.src_pitch = src_pitch;
         H.dst = tmp;
         H.dst_pitch = tmp_pitch;
-        if (R.npad <= kMaxHPad) {
-            H.tiles_x = (w + 511) / 512;
-            H.tiles = tiles_blur_h(w, h);
-            hj.push_back(H);
+        const int px = w >= 384 ? 8 : 2;
+        const int hrow = 64 * px + R.npad + 8;
+        if ((hrow + (hrow >> 5) + 1) * 4 * (int)sizeof(float) <= 65536) {
+            H.tiles_x = (w + 64 * px - 1) / (64 * px);
+            H.tiles = H.tiles_x * ((h + 3) / 4);
+            (px == 8 ? hj : hj2).push_back(H);
         } else {
             H.tiles_x = (w + 255) / 256;
             H.tiles = H.tiles_x * h;
@@ -525,32 +532,48 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         V.dog = dog;
         V.prev = prev;
         V.dog_pitch = dog_pitch;
-        V.tiles_x = (w + 255) / 256;
-        V.tiles = V.tiles_x * ((h + 31) / 32);
-        vj.push_back(V);
+        const int r = R.ntaps / 2;
+        const int narrow_rows = std::min(h, 16 + 2 * r);
+        if (w >= 192 && R.ntaps <= 64) {
+            V.tiles_x = (w + 255) / 256;
+            V.tiles = V.tiles_x * ((h + 31) / 32);
+            vj.push_back(V);
+        } else if (narrow_rows * 64 <= 65536) {
+            V.tiles_x = (w + 63) / 64;
+            V.tiles = V.tiles_x * ((h + 15) / 16);
+            vnj.push_back(V);
+            vn_rows = std::max(vn_rows, narrow_rows);
+        } else {
+            V.tiles_x = (w + 255) / 256;
+            V.tiles = V.tiles_x * h;
+            vgj.push_back(V);
+        }
     };
 
-    auto flush_blur = [&](std::vector<BlurJob>& hj, std::vector<BlurJob>& hgj,
-                          std::vector<BlurJob>& vj, std::vector<BlurJob>& cj) {
-        int maxpad = 0;
+    auto flush_blur = [&]() {
+        int maxpad = 0, maxpad2 = 0;
         for (auto& j : hj) maxpad = std::max(maxpad, j.ntaps_pad);
-        push_launch(0, hj, maxpad);
-        push_launch(3, hgj, 0);
-        push_launch(1, vj, 0);
-        push_launch(2, cj, 0);
-        hj.clear(); hgj.clear(); vj.clear(); cj.clear();
+        for (auto& j : hj2) maxpad2 = std::max(maxpad2, j.ntaps_pad);
+        push_launch(K_BLUR_H, hj, maxpad, 8);
+        push_launch(K_BLUR_H, hj2, maxpad2, 2);
+        push_launch(K_BLUR_H_GENERIC, hgj, 0, 0);
+        push_launch(K_BLUR_V, vj, 0, 0);
+        push_launch(K_BLUR_V_NARROW, vnj, 0, vn_rows);
+        push_launch(K_BLUR_V_GENERIC, vgj, 0, 0);
+        push_launch(K_COPY, cj, 0, 0);
+        hj.clear(); hj2.clear(); hgj.clear(); vj.clear(); vnj.clear(); vgj.clear(); cj.clear();
+        vn_rows = 0;
     };
 
-    std::vector<BlurJob> hj, hgj, vj, cj;
     // initial blur into octave 0 level 0
     for (int i = 0; i < n; i++) {
         const OctShape& s = shapes[i][0];
         const uint8_t* src = B.device_input ? B.images[i].data : (const uint8_t*)AP(in_off[i]);
         make_blur_pair(src, in_pitch[i], (uint8_t*)AP(htemp_off[i]), s.pitch,
                        (uint8_t*)AP(lev_off[i][0]), s.pitch, s.w, s.h, ptab[i]->initial, nullptr,
-                       nullptr, 0, hj, hgj, vj, cj);
+                       nullptr, 0);
     }
-    flush_blur(hj, hgj, vj, cj);
+    flush_blur();
     for (int o = 0; o < maxO; o++) {
         for (int s = 1; s < Lmax; s++) {
             for (int i = 0; i < n; i++) {
@@ -563,9 +586,9 @@ int Context::run(const Batch& B, RunOutputs* outs) {
                 float* dog = B.want_dog ? (float*)AP(dog_off[i][o]) + (size_t)(s - 1) * sh.w * sh.h : nullptr;
                 make_blur_pair(base + (s - 1) * lsz, sh.pitch, (uint8_t*)AP(htemp_off[i]), sh.pitch,
                                base + s * lsz, sh.pitch, sh.w, sh.h, ptab[i]->level[(size_t)o * L + s],
-                               dog, base + (s - 1) * lsz, sh.w, hj, hgj, vj, cj);
+                               dog, base + (s - 1) * lsz, sh.w);
             }
-            flush_blur(hj, hgj, vj, cj);
+            flush_blur();
         }
         // Lanczos3 to the next octave base (src/sift.rs:166-173)
         std::vector<ResizeJob> rv, rh;
@@ -593,7 +616,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             V.count = (const int*)(T + LV.count_off);
             V.weights = (const float*)(T + LV.w_off);
             V.maxtaps = LV.maxtaps;
-            V.tiles_x = (a.w + 255) / 256;
+            V.tiles_x = (a.w + 4 * 256 - 1) / (4 * 256);
             V.tiles = V.tiles_x * b.h;
             rv.push_back(V);
             ResizeJob H = J;
@@ -605,8 +628,8 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             H.tiles = H.tiles_x * b.h;
             rh.push_back(H);
         }
-        push_launch(4, rv, 0);
-        push_launch(5, rh, 0);
+        push_launch(K_RESIZE_V, rv, 0, 0);
+        push_launch(K_RESIZE_H, rh, 0, 0);
     }
 
     // octave descriptors and per-image params
@@ -655,12 +678,22 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         hipEvent_t ev = nullptr;
         prof_begin(L.kind, L.bytes, &ev);
         switch (L.kind) {
-            case 0: launch_blur_h((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.max_pad, st); break;
-            case 1: launch_blur_v((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
-            case 2: launch_copy_level((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
-            case 3: launch_blur_h_generic((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
-            case 4: launch_resize_v((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
-            case 5: launch_resize_h((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
+            case K_BLUR_H:
+                launch_blur_h((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.max_pad, L.variant, st);
+                break;
+            case K_BLUR_V: launch_blur_v((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, 0, st); break;
+            case K_BLUR_V_NARROW:
+                launch_blur_v((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.variant, st);
+                break;
+            case K_BLUR_V_GENERIC:
+                launch_blur_v_generic((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st);
+                break;
+            case K_COPY: launch_copy_level((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
+            case K_BLUR_H_GENERIC:
+                launch_blur_h_generic((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st);
+                break;
+            case K_RESIZE_V: launch_resize_v((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
+            case K_RESIZE_H: launch_resize_h((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
         }
         prof_end(L.kind, L.bytes, ev);
     }
@@ -688,7 +721,11 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             ext_bytes += (double)d.nlevels * d.w * d.h + (double)(d.nlevels - 3) * d.h * d.nwords * 8.0;
         hipEvent_t ev = nullptr;
         prof_begin(K_EXTREMA, ext_bytes, &ev);
-        launch_extrema(d_oct, noct, total_rows, d_par, (unsigned long long*)AP(bits_off), d_counts, st);
+        if (tile_ext)
+            launch_extrema_tile(S0, d_oct, noct, total_tiles, d_par, (unsigned long long*)AP(bits_off),
+                                d_counts, nslots, st);
+        else
+            launch_extrema(d_oct, noct, total_rows, d_par, (unsigned long long*)AP(bits_off), d_counts, st);
         prof_end(K_EXTREMA, ext_bytes, ev);
         prof_begin(K_SCAN, 0.0, &ev);
         launch_scan_dev(d_counts, nslots, nullptr, d_offs, (unsigned*)AP(scantmp1_off), st);

@@ -18,6 +18,8 @@ namespace {
 
 constexpr float kPi = 3.14159274101257324219f;  // std::f32::consts::PI
 constexpr float kTwoPi = 2.0f * kPi;
+constexpr float k8OverTwoPi = 8.0f / kTwoPi;
+constexpr float kDescFix = 2097152.0f;  // 2^21: u32 fixed point of the descriptor sums
 constexpr int kThreads = 256;
 constexpr int kExtChunk = 1024;  // pixels of one row per chunk in k_extrema
 constexpr int kScanTile = 4096;  // 256 threads x 16 elements
@@ -37,6 +39,31 @@ __device__ __forceinline__ float rem_euclid(float x, float m) {
 }
 
 __device__ __forceinline__ float pow2f(int o) { return ldexpf(1.0f, o); }
+
+// atan2 for the gradient angles: octant reduction a = min(|x|,|y|)/max(|x|,|y|),
+// atan(a) = a * P(a^2) with P a degree-8 least-squares (Lawson-reweighted)
+// fit to atan(a)/a on [0,1], evaluated with FMA (max error 1.0e-7 rad on [0,1];
+// about 2e-7 rad after the quadrant corrections, comparable to a 1-2 ulp
+// atan2f). Gradients here are differences of non-negative values, never -0.
+__device__ __forceinline__ float atan2_grad(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float a = mx > 0.0f ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
+    const float s = a * a;
+    float p = 0.002456706715747714f;
+    p = __builtin_fmaf(p, s, -0.014401290565729141f);
+    p = __builtin_fmaf(p, s, 0.0397811196744442f);
+    p = __builtin_fmaf(p, s, -0.07234849780797958f);
+    p = __builtin_fmaf(p, s, 0.1049894317984581f);
+    p = __builtin_fmaf(p, s, -0.14161229133605957f);
+    p = __builtin_fmaf(p, s, 0.19985906779766083f);
+    p = __builtin_fmaf(p, s, -0.33332598209381104f);
+    p = __builtin_fmaf(p, s, 0.9999998807907104f);
+    float r = a * p;
+    if (ay > ax) r = 1.57079637050628662109f - r;
+    if (x < 0.0f) r = 3.14159274101257324219f - r;
+    return copysignf(r, y);
+}
 
 __device__ __forceinline__ void fill_q(float* q) {
     for (int i = threadIdx.x; i < 256; i += blockDim.x) q[i] = __fdiv_rn((float)i, 255.0f);
@@ -138,6 +165,118 @@ __global__ __launch_bounds__(kThreads) void k_extrema(const OctaveDesc* __restri
         __syncthreads();
         if (threadIdx.x == 0) counts[O.slot_begin + (s - 1) * h + y] = cnt;
     }
+}
+
+// ---------------------------------------------------------------------------
+// k_extrema_tile<S>: the same test on 64x16 tiles. All S+3 Gaussian levels of
+// the tile plus a 1-pixel halo are staged once in LDS as q = G/255 floats; lane
+// l owns column x0+l and walks down 4 rows (one wave per 4 rows). Per row it
+// forms the DoG values of every layer at x-1, x, x+1 and their horizontal max /
+// min; a rolling window of three rows gives the 3x3 max / min of each layer.
+// The 26-neighbour maximum of layer s is then max(3x3 of s-1, 3x3 of s+1,
+// row max above and below in s, left and right in s), so "v greater than every
+// neighbour" is v > that maximum (likewise for the minimum) -- the reference's
+// strict comparisons (src/sift.rs:571-576). The 64 results of a row form one
+// bitmask word; the row-slot counts are integer atomics (order-independent).
+// ---------------------------------------------------------------------------
+constexpr int kTileW = 64, kTileH = 16;
+
+template <int S>
+__global__ __launch_bounds__(kThreads) void k_extrema_tile(const OctaveDesc* __restrict__ oct,
+                                                           int noct,
+                                                           const ImageParams* __restrict__ params,
+                                                           unsigned long long* __restrict__ bits,
+                                                           unsigned* __restrict__ counts) {
+    constexpr int NL = S + 2;  // DoG layers
+    constexpr int NG = S + 3;  // Gaussian levels
+    constexpr int RH = kTileH + 2, RW = kTileW + 2;
+    __shared__ float Q[NG][RH][RW];
+    const int tile = blockIdx.x;
+    const int j = find_by(oct, noct, tile, &OctaveDesc::row_begin);
+    const OctaveDesc& O = oct[j];
+    const ImageParams& P = params[O.img];
+    const int t = tile - O.row_begin;
+    const int w = O.w, h = O.h, border = P.border;
+    const int tiles_x = (w + kTileW - 1) / kTileW;
+    const int x0 = (t % tiles_x) * kTileW, y0 = (t / tiles_x) * kTileH;
+    // Stage q = G/255 for columns [x0-1, x0+65) of rows [y0-1, y0+17) (clamped):
+    // 4-byte words starting at x0-4 (x0 is a multiple of 64, rows are 64-byte
+    // aligned); words inside [0, w) are read with one load. q is formed as
+    // (float)((double)v * (1/255)), equal to v / 255.0f for every byte value.
+    constexpr int NW = (kTileW + 8) / 4;  // words covering [x0-4, x0+68)
+    for (int i = threadIdx.x; i < NG * RH * NW; i += kThreads) {
+        const int l = i / (RH * NW);
+        const int rem = i - l * (RH * NW);
+        const int r = rem / NW, k = rem - r * NW;
+        const int y = min(max(y0 - 1 + r, 0), h - 1);
+        const uint8_t* row = O.levels + (long long)l * O.level_stride + (size_t)y * O.pitch;
+        const int xw = x0 - 4 + 4 * k;
+        unsigned u;
+        if (xw >= 0 && xw + 4 <= w) {
+            u = *(const unsigned*)(row + xw);
+        } else {
+            u = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) u |= (unsigned)row[min(max(xw + b, 0), w - 1)] << (8 * b);
+        }
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int c = 4 * k + b - 3;  // column x0-1+c
+            if (c >= 0 && c < RW)
+                Q[l][r][c] = (float)((double)((u >> (8 * b)) & 0xffu) * (1.0 / 255.0));
+        }
+    }
+    __syncthreads();
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int x = x0 + lane;
+    const bool col_ok = x >= border && x < w - border;
+    float hmx[3][NL], hmn[3][NL], dl[3][NL], dc[3][NL], dr[3][NL];
+    auto load_row = [&](int slot, int r) {
+#pragma unroll
+        for (int l = 0; l < NL; l++) {
+            const float a = Q[l + 1][r][lane] - Q[l][r][lane];
+            const float b = Q[l + 1][r][lane + 1] - Q[l][r][lane + 1];
+            const float c = Q[l + 1][r][lane + 2] - Q[l][r][lane + 2];
+            dl[slot][l] = a;
+            dc[slot][l] = b;
+            dr[slot][l] = c;
+            hmx[slot][l] = fmaxf(a, fmaxf(b, c));
+            hmn[slot][l] = fminf(a, fminf(b, c));
+        }
+    };
+    const int rbase = wave * 4;  // local D row of output row y0 + wave*4 is rbase + 1
+    load_row(0, rbase);
+    load_row(1, rbase + 1);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int prv = k % 3, cur = (k + 1) % 3, nxt = (k + 2) % 3;
+        load_row(nxt, rbase + k + 2);
+        const int y = y0 + wave * 4 + k;
+        const bool row_ok = y >= border && y < h - border;
+#pragma unroll
+        for (int s = 1; s <= S; s++) {
+            float mx = fmaxf(fmaxf(hmx[prv][s - 1], hmx[cur][s - 1]), hmx[nxt][s - 1]);
+            mx = fmaxf(mx, fmaxf(fmaxf(hmx[prv][s + 1], hmx[cur][s + 1]), hmx[nxt][s + 1]));
+            mx = fmaxf(mx, fmaxf(fmaxf(hmx[prv][s], hmx[nxt][s]), fmaxf(dl[cur][s], dr[cur][s])));
+            float mn = fminf(fminf(hmn[prv][s - 1], hmn[cur][s - 1]), hmn[nxt][s - 1]);
+            mn = fminf(mn, fminf(fminf(hmn[prv][s + 1], hmn[cur][s + 1]), hmn[nxt][s + 1]));
+            mn = fminf(mn, fminf(fminf(hmn[prv][s], hmn[nxt][s]), fminf(dl[cur][s], dr[cur][s])));
+            const float v = dc[cur][s];
+            const bool ext = row_ok && col_ok && (v > mx || v < mn);
+            const unsigned long long m = ballot64(ext);
+            if (lane == 0 && y < h) {
+                const long long slot_local = (long long)(s - 1) * h + y;
+                bits[O.bits_begin + slot_local * O.nwords + (x0 >> 6)] = m;
+                if (m) atomicAdd(&counts[O.slot_begin + slot_local], (unsigned)__popcll(m));
+            }
+        }
+    }
+}
+
+__global__ void k_fill_u32(unsigned* __restrict__ p, int n, unsigned v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -390,12 +529,19 @@ __global__ __launch_bounds__(kThreads) void k_refine(const OctaveDesc* __restric
 }
 
 // ---------------------------------------------------------------------------
-// k_orient: one wave (workgroup of 64) per refined keypoint, grid-stride.
-// Lanes sweep the (2R+1)^2 window; each sample adds mag * weight to its bin in
-// an LDS int64 fixed-point histogram (order-independent, deterministic). Lanes
-// 0..35 then hold one bin each for the two [1,1,1]/3 smoothing passes, the max,
-// and the peak test; a ballot ranks the peaks in bin order.
+// k_orient: one wave (workgroup of 64) per refined keypoint, grid-stride
+// (src/sift.rs:605-726). The weights exp(-(dx^2+dy^2)/denom) are tabulated by
+// dx^2+dy^2 (the same expf input as the reference). The 64 lanes cover the
+// (2R+1)^2 window on an 8 x 8 lattice, each stepping by 8 in x and y, and add
+// mag * weight to an LDS
+// int64 fixed-point histogram, one of 8 copies by lane (integer adds:
+// order-independent, deterministic; the copies are summed afterwards).
+// Lanes 0..35 then hold one bin each for the two
+// [1,1,1]/3 smoothing passes, the max and the peak test; a ballot ranks the
+// peaks in bin order.
 // ---------------------------------------------------------------------------
+constexpr int kOriMaxR = 16;
+
 __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oct,
                                                const ImageParams* __restrict__ params,
                                                const Cand* __restrict__ cand,
@@ -405,7 +551,8 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                                                unsigned* __restrict__ npeaks,
                                                float* __restrict__ angles) {
     __shared__ float q[256];
-    __shared__ unsigned long long hist[36];
+    __shared__ unsigned long long hist[8][36];  // 8 copies (lane & 7): fewer atomic conflicts
+    __shared__ float wtab[2 * kOriMaxR * kOriMaxR + 1];
     fill_q(q);
     __syncthreads();
     const int lane = threadIdx.x;
@@ -431,32 +578,45 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
         const int R = f2i_sat(roundf(4.5f * so));
         const float ws = 1.5f * so;
         const float denom = 2.0f * ws * ws;
-        if (lane < 36) hist[lane] = 0ull;
+        const bool table = R <= kOriMaxR;
+        for (int i = lane; i < 8 * 36; i += 64) (&hist[0][0])[i] = 0ull;
+        if (table)
+            for (int d = lane; d <= 2 * R * R; d += 64) wtab[d] = expf(-(float)d / denom);
         __syncthreads();
-        const int side = 2 * R + 1;
-        const int nsamp = side * side;
-        for (int idx = lane; idx < nsamp; idx += 64) {
-            const int iy = idx / side;
-            const int dy = iy - R, dx = idx - iy * side - R;
-            const int xi = f2i_sat(roundf(xo + (float)dx));
+        // lane (ly, lx) = (l / 8, l % 8) takes the samples dy = ly - R + 8i, dx = lx - R + 8j
+        for (int dy = (lane >> 3) - R; dy <= R; dy += 8) {
             const int yi = f2i_sat(roundf(yo + (float)dy));
-            if (xi < 1 || xi >= w - 1 || yi < 1 || yi >= h - 1) continue;
-            const uint8_t* p = g + (size_t)yi * pitch + xi;
-            const float gx = q[p[1]] - q[p[-1]];
-            const float gy = q[p[pitch]] - q[p[-pitch]];
-            const float mag = sqrtf(gx * gx + gy * gy);
-            const float ang = atan2f(gy, gx);
-            const float fdx = (float)dx, fdy = (float)dy;
-            const float wt = expf(-(fdx * fdx + fdy * fdy) / denom);
-            const float an = ang < 0.0f ? ang + kTwoPi : ang;
-            const float bf = an * 36.0f / kTwoPi;
-            const int bin = f2i_sat(floorf(bf)) % 36;
-            const float contrib = mag * wt;
-            atomicAdd(&hist[bin], (unsigned long long)(double)(contrib * (float)kFixScale));
+            if (yi < 1 || yi >= h - 1) continue;
+            const uint8_t* grow = g + (size_t)yi * pitch;
+            for (int dx = (lane & 7) - R; dx <= R; dx += 8) {
+                const int xi = f2i_sat(roundf(xo + (float)dx));
+                if (xi < 1 || xi >= w - 1) continue;
+                const uint8_t* p = grow + xi;
+                const float gx = q[p[1]] - q[p[-1]];
+                const float gy = q[p[pitch]] - q[p[-pitch]];
+                float wt;
+                if (table) {
+                    wt = wtab[dx * dx + dy * dy];
+                } else {
+                    const float fdx = (float)dx, fdy = (float)dy;
+                    wt = expf(-(fdx * fdx + fdy * fdy) / denom);
+                }
+                const float mag = sqrtf(gx * gx + gy * gy);
+                const float ang = atan2_grad(gy, gx);
+                const float an = ang < 0.0f ? ang + kTwoPi : ang;
+                const float bf = an * 36.0f / kTwoPi;
+                const int bin = ((int)floorf(bf)) % 36;
+                atomicAdd(&hist[lane & 7][bin], (unsigned long long)(double)(mag * wt * (float)kFixScale));
+            }
         }
         __syncthreads();
         float hv = 0.0f;
-        if (lane < 36) hv = (float)((double)hist[lane] * (1.0 / kFixScale));
+        if (lane < 36) {
+            unsigned long long sum = 0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) sum += hist[k][lane];
+            hv = (float)((double)sum * (1.0 / kFixScale));
+        }
         for (int it = 0; it < 2; it++) {
             const float pv = __shfl(hv, lane < 36 ? (lane + 35) % 36 : lane, 64);
             const float nv = __shfl(hv, lane < 36 ? (lane + 1) % 36 : lane, 64);
@@ -488,11 +648,7 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
 }
 
 // ---------------------------------------------------------------------------
-// k_descriptor: one workgroup (256) per refined keypoint, looping over its
-// orientation peaks. The (2R+1)^2 image-grid samples are spread over the
-// threads; each in-window sample adds its 8 trilinear contributions to an LDS
-// int64 fixed-point 4x4x8 histogram. Wave 0 normalises (norm, clip 0.2,
-// renormalise) and writes the keypoint and its 128 floats.
+// Descriptor helpers: clamped u8 sample, get_gauss_pixel_bilinear, wave sum.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float gauss_px(const uint8_t* g, int pitch, int w, int h, int x, int y,
                                           const float* q) {
@@ -522,16 +678,79 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
+constexpr int kDescMaxR = 31;                         // sigma_octave <= 3.65
+constexpr int kDescGrid = (2 * kDescMaxR + 3) * (2 * kDescMaxR + 3);  // bilinear grid
+
+// Trilinear scatter of one sample (src/sift.rs:1012-1049) into a u32 fixed-point
+// histogram padded to 6x6 spatial cells (cells -1..4 in x and y) x 8 angles:
+// the reference drops neighbours outside the 4x4 grid; here they land in the
+// border cells, which are never read, so no clamping or branching is needed and
+// the four spatial neighbours are constant offsets (LDS instruction offsets).
+// Contribution ((wm * wx) * wy) * wa in the reference's product order; wm_fix =
+// weighted magnitude * 2^21; xb, yb in (-1, 4), ab in [0, 8]. Integer adds make
+// the sums independent of the order.
+constexpr int kDescPadBins = 6 * 6 * 8;
+
+__device__ __forceinline__ int desc_bin(int iy, int ix, int ia) {  // ix, iy in 0..3
+    return ((iy + 1) * 6 + (ix + 1)) * 8 + ia;
+}
+
+__device__ __forceinline__ void desc_scatter(unsigned* hist, float xb, float yb, float ab,
+                                             float wm_fix) {
+    const float xf = floorf(xb), yf = floorf(yb), af = floorf(ab);
+    const int xbi = (int)xf, ybi = (int)yf, abi = (int)af;
+    const float dxi = xb - xf, dyi = yb - yf, dai = ab - af;
+    const float wx[2] = {1.0f - dxi, dxi};
+    const float wy[2] = {1.0f - dyi, dyi};
+    const float wa[2] = {1.0f - dai, dai};
+    unsigned* cell = hist + ((ybi + 1) * 6 + (xbi + 1)) * 8;
+    const int ia[2] = {abi & 7, (abi + 1) & 7};
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const float t = wm_fix * wx[i];
+#pragma unroll
+        for (int jj = 0; jj < 2; jj++) {
+            const float tt = t * wy[jj];
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+                atomicAdd(&cell[(jj * 6 + i) * 8 + ia[k]], (unsigned)(tt * wa[k] + 0.5f));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_descriptor: one workgroup (256) per refined keypoint, looping over its
+// orientation peaks (src/sift.rs:917-1060).
+// Per keypoint: the clamped u8 patch of the Gaussian level around (x_o, y_o) is
+// staged in LDS (every texel the reference's get_gauss_pixel_bilinear can touch
+// in the window, with its clamp-to-edge rule), and the bilinear samples on the
+// shifted grid (x_o + i, y_o + j), |i|,|j| <= R+1, are formed once from it. A
+// window sample (px, py) takes its gradient from the four grid neighbours
+// (the reference's bilinear(x_s +- 1, y_s) and bilinear(x_s, y_s +- 1)).
+// Per peak, each sample is rotated and binned; in-window samples compute
+// mag * exp(-(px^2+py^2)/(2 (6 sigma_o)^2)) and atan2 and scatter trilinearly
+// into a per-wave u32 fixed-point histogram (integer adds: independent of the
+// order; per-wave sums stay below 2^32 for any window of at most 4096 samples).
+// The four wave histograms are summed and wave 0 normalises, clips at
+// 0.2 and renormalises (src/sift.rs:889-913).
+// Keypoints whose window exceeds the LDS patch (sigma_octave > 3.65) use the
+// direct per-sample path with global reads.
+// ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void k_descriptor(
     const OctaveDesc* __restrict__ oct, const ImageParams* __restrict__ params,
     const Cand* __restrict__ cand, const unsigned* __restrict__ total, int cap,
     const Keypoint* __restrict__ ref, const unsigned* __restrict__ npeaks,
     const float* __restrict__ angles, const unsigned* __restrict__ koff, int kcap,
     Keypoint* __restrict__ kps_out, float* __restrict__ desc_out) {
+    constexpr int kPatch = 2 * kDescMaxR + 6;
     __shared__ float q[256];
-    __shared__ unsigned long long hist[128];
+    __shared__ uint8_t patch[kPatch * kPatch];
+    __shared__ float grid[kDescGrid];
+    __shared__ float ex[kDescMaxR + 2];
+    __shared__ unsigned hist[4][kDescPadBins];
     fill_q(q);
     __syncthreads();
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int n = (int)min(*total, (unsigned)cap);
     for (int c = blockIdx.x; c < n; c += gridDim.x) {
         const int np = (int)npeaks[c];
@@ -546,70 +765,134 @@ __global__ __launch_bounds__(kThreads) void k_descriptor(
         const float scale = pow2f(base.octave);
         const float xo = base.x / scale, yo = base.y / scale, so = base.size / scale;
         const unsigned k0 = koff[c];
+        const float bw = 3.0f * so;
+        const float win = bw * 4.0f;
+        const float wsig = 0.5f * win;
+        const float denom = 2.0f * wsig * wsig;
+        const int R = so > 0.0f ? f2i_sat(ceilf(win * sqrtf(2.0f) * 0.5f)) : 0;
+        const bool lds_path = so > 0.0f && R <= kDescMaxR;
+        const int side = 2 * R + 1, gside = 2 * R + 3;
+        const int nsamp = side * side;
+        const float wm1 = (float)(w - 1), hm1 = (float)(h - 1);
+        if (lds_path) {
+            // patch texel (u, v) = G(clamp(px0 + u), clamp(py0 + v)); every bilinear
+            // tap of the grid, floor(x_o +- (R+1)) .. +1, lies in [px0, px0 + pside)
+            const int pside = 2 * R + 6;
+            const int px0 = f2i_sat(floorf(xo)) - R - 2, py0 = f2i_sat(floorf(yo)) - R - 2;
+            for (int i = tid; i < pside * pside; i += kThreads) {
+                const int v = i / pside, u = i - v * pside;
+                const int xx = min(max(px0 + u, 0), w - 1), yy = min(max(py0 + v, 0), h - 1);
+                patch[i] = g[(size_t)yy * pitch + xx];
+            }
+            __syncthreads();
+            for (int i = tid; i < gside * gside; i += kThreads) {
+                const int gy = i / gside, gx = i - gy * gside;
+                const float x = xo + (float)(gx - R - 1), y = yo + (float)(gy - R - 1);
+                const float xf = floorf(x), yf = floorf(y);
+                const float dx = x - xf, dy = y - yf;
+                const int u0 = min(max(f2i_sat(xf) - px0, 0), pside - 2);
+                const int v0 = min(max(f2i_sat(yf) - py0, 0), pside - 2);
+                const uint8_t* pp = patch + v0 * pside + u0;
+                const float q11 = q[pp[0]], q21 = q[pp[1]];
+                const float q12 = q[pp[pside]], q22 = q[pp[pside + 1]];
+                grid[i] = q11 * (1.0f - dx) * (1.0f - dy) + q21 * dx * (1.0f - dy) +
+                          q12 * (1.0f - dx) * dy + q22 * dx * dy;
+            }
+        }
+        const int per = (nsamp + kThreads - 1) / kThreads;
+        const int chunk0 = (lane * 4 + wave) * per;  // lanes of a wave spread over the window
+        // Gaussian weight exp(-(px^2+py^2)/denom) as exp(-px^2/denom) * exp(-py^2/denom)
+        if (lds_path)
+            for (int i = tid; i <= R; i += kThreads) ex[i] = expf(-(float)(i * i) / denom);
+        // (ix, iy) range whose x_o + (ix - R) lies in [0, w-1) and y_o + (iy - R) in [0, h-1),
+        // evaluated with the reference's f32 test at the candidate integers
+        int bx0 = 0, bx1 = side - 1, by0 = 0, by1 = side - 1;
+        while (bx0 <= bx1 && xo + (float)(bx0 - R) < 0.0f) bx0++;
+        while (bx1 >= bx0 && xo + (float)(bx1 - R) >= wm1) bx1--;
+        while (by0 <= by1 && yo + (float)(by0 - R) < 0.0f) by0++;
+        while (by1 >= by0 && yo + (float)(by1 - R) >= hm1) by1--;
         for (int pk = 0; pk < np; pk++) {
             const float angle = angles[(size_t)c * kMaxPeaks + pk];
-            if (threadIdx.x < 128) hist[threadIdx.x] = 0ull;
+            for (int i = tid; i < 4 * kDescPadBins; i += kThreads) (&hist[0][0])[i] = 0u;
             __syncthreads();
             if (so > 0.0f) {
                 const float ca = cosf(angle), sa = sinf(angle);
-                const float bw = 3.0f * so;
-                const float win = bw * 4.0f;
-                const float wsig = 0.5f * win;
-                const float denom = 2.0f * wsig * wsig;
-                const int R = f2i_sat(ceilf(win * sqrtf(2.0f) * 0.5f));
-                const float wm1 = (float)(w - 1), hm1 = (float)(h - 1);
-                const int side = 2 * R + 1;
-                const int nsamp = side * side;
-                for (int idx = threadIdx.x; idx < nsamp; idx += kThreads) {
-                    const int iy = idx / side;
-                    const float py = (float)(iy - R), px = (float)(idx - iy * side - R);
-                    const float rx = ca * px + sa * py;
-                    const float ry = -sa * px + ca * py;
-                    const float xb = rx / bw + 4.0f / 2.0f - 0.5f;
-                    const float yb = ry / bw + 4.0f / 2.0f - 0.5f;
-                    if (!(xb > -1.0f && xb < 4.0f && yb > -1.0f && yb < 4.0f)) continue;
-                    const float xs = xo + px, ys = yo + py;
-                    if (xs < 0.0f || xs >= wm1 || ys < 0.0f || ys >= hm1) continue;
-                    const float gx = bilinear(g, pitch, w, h, xs + 1.0f, ys, q) -
-                                     bilinear(g, pitch, w, h, xs - 1.0f, ys, q);
-                    const float gy = bilinear(g, pitch, w, h, xs, ys + 1.0f, q) -
-                                     bilinear(g, pitch, w, h, xs, ys - 1.0f, q);
-                    const float mag = sqrtf(gx * gx + gy * gy);
-                    const float pa = atan2f(gy, gx);
-                    const float ar = rem_euclid(pa - angle, kTwoPi);
-                    const float wt = expf(-(px * px + py * py) / denom);
-                    const float wm = mag * wt;
-                    const float ab = ar * 8.0f / kTwoPi;
-                    const int xbi = f2i_sat(floorf(xb)), ybi = f2i_sat(floorf(yb)),
-                              abi = f2i_sat(floorf(ab));
-                    const float dxi = xb - (float)xbi, dyi = yb - (float)ybi, dai = ab - (float)abi;
-#pragma unroll
-                    for (int i = 0; i < 2; i++) {
-#pragma unroll
-                        for (int jj = 0; jj < 2; jj++) {
-#pragma unroll
-                            for (int k = 0; k < 2; k++) {
-                                const int ix = xbi + i, iy2 = ybi + jj;
-                                int ia = (abi + k) % 8;
-                                if (ia < 0) ia += 8;
-                                if (ix >= 0 && ix < 4 && iy2 >= 0 && iy2 < 4) {
-                                    const float wx = i == 0 ? 1.0f - dxi : dxi;
-                                    const float wy = jj == 0 ? 1.0f - dyi : dyi;
-                                    const float wa = k == 0 ? 1.0f - dai : dai;
-                                    const float cc = wm * wx * wy * wa;
-                                    atomicAdd(&hist[(iy2 * 4 + ix) * 8 + ia],
-                                              (unsigned long long)(double)(cc * (float)kFixScale));
-                                }
-                            }
+                const float inv_bw = 1.0f / bw;
+                unsigned* hw = hist[wave];
+                if (lds_path) {
+                    // Lane l owns block (l / 8, l % 8) of an 8 x 8 split of the window;
+                    // wave w takes rows w, w+4, ... of each block, so at every step the
+                    // 64 lanes sit on a lattice spread over the whole window (different
+                    // histogram cells). Samples with x_o + px in [0, w-1) and y_o + py in
+                    // [0, h-1) (src/sift.rs:981-987) form the rectangle [bx0,bx1] x
+                    // [by0,by1], which bounds the loops. -sin * px is computed as
+                    // (-sin) * px, the same value.
+                    const int rs = (side + 7) >> 3;
+                    const int oy = (lane >> 3) * rs, oxx = (lane & 7) * rs;
+                    const int kx0 = max(0, bx0 - oxx), kx1 = min(rs - 1, bx1 - oxx);
+                    const float msa = -sa;
+                    for (int ky = wave; ky < rs; ky += 4) {
+                        const int cy = oy + ky;
+                        if (cy < by0 || cy > by1) continue;
+                        const float py = (float)(cy - R);
+                        const float sy = sa * py, cy_ = ca * py;  // per-row terms of rx, ry
+                        const float wy = ex[abs(cy - R)];
+                        const float* gr = grid + (cy + 1) * gside + (oxx + kx0 + 1);
+                        for (int kx = kx0; kx <= kx1; kx++, gr++) {
+                            const int cx = oxx + kx;
+                            const float px = (float)(cx - R);
+                            const float rx = ca * px + sy;
+                            const float ry = msa * px + cy_;
+                            const float xb = rx * inv_bw + 4.0f / 2.0f - 0.5f;
+                            const float yb = ry * inv_bw + 4.0f / 2.0f - 0.5f;
+                            if (!(xb > -1.0f && xb < 4.0f && yb > -1.0f && yb < 4.0f)) continue;
+                            const float gx = gr[1] - gr[-1];
+                            const float gyv = gr[gside] - gr[-gside];
+                            const float wt = ex[abs(cx - R)] * wy;
+                            const float mag = __builtin_amdgcn_sqrtf(gx * gx + gyv * gyv);
+                            const float pa = atan2_grad(gyv, gx);
+                            const float d = pa - angle;  // in [-2pi, 2pi): rem_euclid(d, 2pi)
+                            const float ar = d < 0.0f ? d + kTwoPi : d;
+                            desc_scatter(hw, xb, yb, ar * k8OverTwoPi, mag * wt * kDescFix);
                         }
+                    }
+                } else {
+                    const int iend = min(chunk0 + per, nsamp);
+                    for (int i = chunk0; i < iend; i++) {
+                        const int cy = i / side, cx = i - cy * side;
+                        const float py = (float)(cy - R), px = (float)(cx - R);
+                        const float rx = ca * px + sa * py;
+                        const float ry = -sa * px + ca * py;
+                        const float xb = rx / bw + 4.0f / 2.0f - 0.5f;
+                        const float yb = ry / bw + 4.0f / 2.0f - 0.5f;
+                        if (!(xb > -1.0f && xb < 4.0f && yb > -1.0f && yb < 4.0f)) continue;
+                        const float xs = xo + px, ys = yo + py;
+                        if (xs < 0.0f || xs >= wm1 || ys < 0.0f || ys >= hm1) continue;
+                        const float gx = bilinear(g, pitch, w, h, xs + 1.0f, ys, q) -
+                                         bilinear(g, pitch, w, h, xs - 1.0f, ys, q);
+                        const float gy = bilinear(g, pitch, w, h, xs, ys + 1.0f, q) -
+                                         bilinear(g, pitch, w, h, xs, ys - 1.0f, q);
+                        const float wt = expf(-(px * px + py * py) / denom);
+                        const float mag = sqrtf(gx * gx + gy * gy);
+                        const float pa = atan2f(gy, gx);
+                        const float ar = rem_euclid(pa - angle, kTwoPi);
+                        desc_scatter(hw, xb, yb, ar * 8.0f / kTwoPi, mag * wt * kDescFix);
                     }
                 }
             }
             __syncthreads();
-            if (threadIdx.x < 64) {
-                const int l = threadIdx.x;
-                float v0 = (float)((double)hist[2 * l] * (1.0 / kFixScale));
-                float v1 = (float)((double)hist[2 * l + 1] * (1.0 / kFixScale));
+            if (tid < 64) {
+                const int l = tid;
+                float v[2];
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    const int d = 2 * l + e;  // (iy * 4 + ix) * 8 + ia
+                    const int b = desc_bin(d >> 5, (d >> 3) & 3, d & 7);
+                    const unsigned long long sum = (unsigned long long)hist[0][b] + hist[1][b] +
+                                                   hist[2][b] + hist[3][b];
+                    v[e] = (float)((double)sum * (1.0 / (double)kDescFix));
+                }
+                float v0 = v[0], v1 = v[1];
                 const float nrm = sqrtf(wave_sum(v0 * v0 + v1 * v1));
                 if (nrm >= 1e-8f) {
                     const float inv = 1.0f / nrm;
@@ -624,8 +907,8 @@ __global__ __launch_bounds__(kThreads) void k_descriptor(
                 }
                 const unsigned kidx = k0 + (unsigned)pk;
                 if (kidx < (unsigned)kcap) {
-                    float2* d = (float2*)(desc_out + (size_t)kidx * 128);
-                    d[l] = make_float2(v0, v1);
+                    float2* dd = (float2*)(desc_out + (size_t)kidx * 128);
+                    dd[l] = make_float2(v0, v1);
                     if (l == 0) {
                         Keypoint o = base;
                         o.angle = angle;
@@ -662,6 +945,24 @@ void launch_extrema(const OctaveDesc* d_oct, int noct, int total_rows, const Ima
     if (total_rows <= 0) return;
     hipLaunchKernelGGL(k_extrema, dim3(total_rows), dim3(kThreads), 0, s, d_oct, noct, d_params,
                        d_bits, d_counts);
+}
+
+bool extrema_tile_supported(int S) { return S >= 1 && S <= 6; }
+
+void launch_extrema_tile(int S, const OctaveDesc* d_oct, int noct, int total_tiles,
+                         const ImageParams* d_params, unsigned long long* d_bits, unsigned* d_counts,
+                         int nslots, hipStream_t s) {
+    if (nslots > 0)
+        hipLaunchKernelGGL(k_fill_u32, dim3((nslots + 255) / 256), dim3(256), 0, s, d_counts, nslots, 0u);
+    if (total_tiles <= 0) return;
+    switch (S) {
+#define SH_EXT(SS) \
+        case SS: hipLaunchKernelGGL(k_extrema_tile<SS>, dim3(total_tiles), dim3(kThreads), 0, s, d_oct, \
+                                    noct, d_params, d_bits, d_counts); break;
+        SH_EXT(1) SH_EXT(2) SH_EXT(3) SH_EXT(4) SH_EXT(5) SH_EXT(6)
+#undef SH_EXT
+        default: break;
+    }
 }
 
 // n_dev may be null (then n_max is the count). out needs n_max+1 entries;

@@ -4,18 +4,10 @@
 //                         (src/sift.rs:92-124,182-204) in the vertical pass.
 //   k_resize_v/_h       : image resize_exact(.., Lanczos3) (src/sift.rs:83-88).
 //
-// Exactness of the blur. The reference computes, per output pixel,
-//   s = 0; for i in taps: s = s + p_i * w_i      (f32, no FMA, in tap order)
-// then truncates s to u8. Every p_i is an integer in [0,255] and every w_i > 0,
-// so all terms are non-negative. The kernel first computes t with one fused
-// multiply-add per tap (single rounding each step). Standard recursive-summation
-// bounds give |s - S| <= g(n+1) S and |t - S| <= g(n) S with S the exact sum and
-// g(k) = k u / (1 - k u), u = 2^-24, so |s - t| <= 2 g(n+1) S <= eps_rel * t
-// where eps_rel = 2.05 (n+2) u is a host-computed constant (with margin). If the
-// u8 truncations of t - E and t + E agree (E = eps_rel t + 6.2e-5; the absolute
-// term covers the rounding of t +- E), trunc(s) is that value; otherwise the
-// kernel recomputes s exactly in the reference order (mul, then add) for that
-// pixel. The result equals the reference's u8 for every pixel.
+// Exactness: every pass computes, per output pixel, acc = 0; acc = acc + p_i * w_i
+// in tap order with separate multiply and add (no FMA; the file is compiled with
+// -ffp-contract=off and uses __fmul_rn/__fadd_rn), then truncates to u8 -- the
+// reference's arithmetic, so the levels are bit-identical to it.
 #include "sift_internal.h"
 
 namespace sh {
@@ -23,7 +15,6 @@ namespace sh {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kHTileW = 512;  // 64 lanes x 8 px
 constexpr int kHTileH = 4;    // one row per wave
 constexpr int kVTileW = 256;  // 64 lanes x 4 px
 constexpr int kVTileH = 32;   // 4 waves x 8 rows
@@ -49,110 +40,147 @@ __device__ __forceinline__ float mul_add_exact(float acc, float a, float b) {
     return __fadd_rn(acc, __fmul_rn(a, b));
 }
 
-// Decides trunc(s) from the FMA sum t, or reports that it is ambiguous.
-__device__ __forceinline__ bool fast_u8(float t, float eps_rel, unsigned& out) {
-    float e = t * eps_rel + kAbsMargin;
-    unsigned lo = u8_trunc(t - e), hi = u8_trunc(t + e);
-    out = lo;
-    return lo == hi;
-}
-
 __device__ __forceinline__ float q255(unsigned v) { return __fdiv_rn((float)v, 255.0f); }
 
 // ---------------------------------------------------------------------------
-// Horizontal pass. Tile = 512 columns x 4 rows; wave w computes row y0+w; lane l
-// computes the 8 outputs x0+8l .. x0+8l+7 from a register window of 16 inputs
-// that slides over the taps in blocks of 8 (64 FMAs per 2 LDS reads). Taps are
-// wave-uniform (scalar registers). The row segment [x0-r, x0+512+pad) is staged
-// in LDS as f32 with clamp-to-edge indexing, which is the reference's edge rule.
+// Horizontal pass. Tile = (64*PX) columns x 4 rows; wave w computes row y0+w;
+// lane l computes PX consecutive outputs. The row segment [x0-r, x0+64*PX+pad)
+// is staged in LDS as f32 with clamp-to-edge indexing (the reference's edge
+// rule), with one pad word after every 32 (element c at c + c/32) so that the
+// lanes' reads, 8 elements apart, fall in distinct banks. A register window of
+// PX+4 inputs slides over the taps in blocks of 4; taps are zero-padded to a
+// multiple of 4 (p * 0 added to a non-negative sum leaves it unchanged) and
+// are wave-uniform (scalar registers).
 // ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ int hpad(int c) { return c + (c >> 5); }
+
+template <int PX>
 __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__ jobs, int njobs) {
     extern __shared__ float lds[];
+    constexpr int TW = 64 * PX;
     const int tile = blockIdx.x;
     const int j = find_job(jobs, njobs, tile);
     const BlurJob& J = jobs[j];
     const int t = tile - J.tile_begin;
     const int tx = t % J.tiles_x, ty = t / J.tiles_x;
-    const int x0 = tx * kHTileW, y0 = ty * kHTileH;
+    const int x0 = tx * TW, y0 = ty * kHTileH;
     const int w = J.w, h = J.h;
     const int r = J.ntaps >> 1;
     const int npad = J.ntaps_pad;
-    const int rowlen = kHTileW + npad + 8;
+    const int rowlen = TW + npad + 8;
+    const int rowstride = hpad(rowlen) + 1;
 
-    for (int rr = 0; rr < kHTileH; rr++) {
+    // Stage rows y0..y0+3 (clamped), columns [x0-r, x0-r+rowlen) (clamped) as f32.
+    // Aligned 4-byte words fully inside [0, w) are read with one load; words at
+    // the image edges (and unaligned sources) fall back to clamped bytes.
+    const int xs = x0 - r;
+    const int ws = xs & ~3;                      // first word start (may be < 0)
+    const int nwords = (xs + rowlen - ws + 3) >> 2;
+    const bool aligned = (((uintptr_t)J.src | (uintptr_t)J.src_pitch) & 3) == 0;
+    for (int it = threadIdx.x; it < kHTileH * nwords; it += kThreads) {
+        const int rr = it / nwords, k = it - rr * nwords;
         const int y = min(y0 + rr, h - 1);
         const uint8_t* src = J.src + (size_t)y * J.src_pitch;
-        float* dstrow = lds + rr * rowlen;
-        for (int c = threadIdx.x; c < rowlen; c += kThreads) {
-            int x = min(max(x0 - r + c, 0), w - 1);
-            dstrow[c] = (float)src[x];
+        float* dstrow = lds + rr * rowstride;
+        const int xw = ws + 4 * k;
+        const int c0 = xw - xs;
+        float v[4];
+        if (aligned && xw >= 0 && xw + 4 <= w) {
+            const unsigned u = *(const unsigned*)(src + xw);
+            v[0] = (float)(u & 0xff);
+            v[1] = (float)((u >> 8) & 0xff);
+            v[2] = (float)((u >> 16) & 0xff);
+            v[3] = (float)(u >> 24);
+        } else {
+#pragma unroll
+            for (int b = 0; b < 4; b++) v[b] = (float)src[min(max(xw + b, 0), w - 1)];
         }
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+            if (c0 + b >= 0 && c0 + b < rowlen) dstrow[hpad(c0 + b)] = v[b];
     }
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int y = y0 + wave;
     if (y >= h) return;
-    const float* row = lds + wave * rowlen + lane * 8;
+    const float* row = lds + wave * rowstride;
+    const int base = lane * PX;
     const float* __restrict__ taps = J.taps;
+    const int n4 = (J.ntaps + 3) & ~3;  // <= ntaps_pad; taps beyond ntaps are zero
 
-    float acc[8];
+    float acc[PX];
 #pragma unroll
-    for (int q = 0; q < 8; q++) acc[q] = 0.0f;
-    float4 lo0 = *(const float4*)(row + 0);
-    float4 lo1 = *(const float4*)(row + 4);
-    for (int b = 0; b < npad; b += 8) {
-        float4 hi0 = *(const float4*)(row + b + 8);
-        float4 hi1 = *(const float4*)(row + b + 12);
-        float win[16] = {lo0.x, lo0.y, lo0.z, lo0.w, lo1.x, lo1.y, lo1.z, lo1.w,
-                         hi0.x, hi0.y, hi0.z, hi0.w, hi1.x, hi1.y, hi1.z, hi1.w};
+    for (int q = 0; q < PX; q++) acc[q] = 0.0f;
+    float win[PX + 4];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < PX; k++) win[k] = row[hpad(base + k)];
+    for (int b = 0; b < n4; b += 4) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) win[PX + k] = row[hpad(base + PX + b + k)];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
             const float wk = taps[b + k];
 #pragma unroll
-            for (int q = 0; q < 8; q++) acc[q] = __builtin_fmaf(win[q + k], wk, acc[q]);
+            for (int q = 0; q < PX; q++) acc[q] = mul_add_exact(acc[q], win[q + k], wk);
         }
-        lo0 = hi0;
-        lo1 = hi1;
-    }
-
-    const float eps = J.eps_rel;
-    unsigned res[8];
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        if (!fast_u8(acc[q], eps, res[q])) {
-            float s = 0.0f;
-            for (int i = 0; i < J.ntaps; i++) s = mul_add_exact(s, row[q + i], taps[i]);
-            res[q] = u8_trunc(s);
-        }
+        for (int k = 0; k < PX; k++) win[k] = win[k + 4];
     }
 
-    const int x = x0 + lane * 8;
+    const int x = x0 + base;
     if (x >= w) return;
     uint8_t* dst = J.dst + (size_t)y * J.dst_pitch + x;
-    if (x + 8 <= w) {
-        uint2 v;
-        v.x = res[0] | (res[1] << 8) | (res[2] << 16) | (res[3] << 24);
-        v.y = res[4] | (res[5] << 8) | (res[6] << 16) | (res[7] << 24);
-        *(uint2*)dst = v;
-    } else {
+    unsigned res[PX];
 #pragma unroll
-        for (int q = 0; q < 8; q++)
-            if (x + q < w) dst[q] = (uint8_t)res[q];
+    for (int q = 0; q < PX; q++) res[q] = u8_trunc(acc[q]);
+    if constexpr (PX == 8) {
+        if (x + 8 <= w) {
+            uint2 v;
+            v.x = res[0] | (res[1] << 8) | (res[2] << 16) | (res[3] << 24);
+            v.y = res[4] | (res[5] << 8) | (res[6] << 16) | (res[7] << 24);
+            *(uint2*)dst = v;
+            return;
+        }
     }
+#pragma unroll
+    for (int q = 0; q < PX; q++)
+        if (x + q < w) dst[q] = (uint8_t)res[q];
 }
 
 // ---------------------------------------------------------------------------
-// Vertical pass. Tile = 256 columns x 32 rows; lane l owns columns x0+4l..+3
-// (one u32 load per input row), wave w owns rows y0+8w..+7. A register window
-// of 16 input rows x 4 columns slides over the taps in blocks of 8 (256 FMAs per
-// 8 loads). Optional DoG epilogue: dog = q[G_s] - q[G_{s-1}], q[v] = v / 255.
+// Vertical pass, wide octaves. Tile = 256 columns x 32 rows; lane l owns columns
+// x0+4l..+3 (one u32 load per input row), wave w owns rows y0+8w..+7. A register
+// window of 16 input rows x 4 columns slides over the taps in blocks of 8.
+// Optional DoG epilogue: dog = q[G_s] - q[G_{s-1}], q[v] = v / 255.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void cvt4(unsigned v, float* o) {
     o[0] = (float)(v & 0xff);
     o[1] = (float)((v >> 8) & 0xff);
     o[2] = (float)((v >> 16) & 0xff);
     o[3] = (float)(v >> 24);
+}
+
+__device__ __forceinline__ void store_row4(const BlurJob& J, int y, int x, const float* acc) {
+    unsigned res[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) res[c] = u8_trunc(acc[c]);
+    uint8_t* dst = J.dst + (size_t)y * J.dst_pitch + x;
+    const int w = J.w;
+    if (x + 4 <= w) {
+        *(unsigned*)dst = res[0] | (res[1] << 8) | (res[2] << 16) | (res[3] << 24);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if (x + c < w) dst[c] = (uint8_t)res[c];
+    }
+    if (J.dog) {
+        const uint8_t* prev = J.prev + (size_t)y * J.dst_pitch + x;
+        float* dg = J.dog + (size_t)y * J.dog_pitch + x;
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if (x + c < w) dg[c] = q255(res[c]) - q255(prev[c]);
+    }
 }
 
 __global__ __launch_bounds__(kThreads) void k_blur_v(const BlurJob* __restrict__ jobs, int njobs) {
@@ -167,7 +195,7 @@ __global__ __launch_bounds__(kThreads) void k_blur_v(const BlurJob* __restrict__
     const int yb = ty * kVTileH + wave * 8;
     if (yb >= h || x >= w) return;
     const int r = J.ntaps >> 1;
-    const int npad = J.ntaps_pad;
+    const int n = J.ntaps;
     const int pitch = J.src_pitch;
     const uint8_t* col = J.src + x;
     const float* __restrict__ taps = J.taps;
@@ -184,7 +212,8 @@ __global__ __launch_bounds__(kThreads) void k_blur_v(const BlurJob* __restrict__
         int yy = min(max(yb - r + i, 0), h - 1);
         cvt4(*(const unsigned*)(col + (size_t)yy * pitch), lo[i]);
     }
-    for (int b = 0; b < npad; b += 8) {
+    int b = 0;
+    for (; b + 8 <= n; b += 8) {
         float hi[8][4];
 #pragma unroll
         for (int i = 0; i < 8; i++) {
@@ -199,7 +228,7 @@ __global__ __launch_bounds__(kThreads) void k_blur_v(const BlurJob* __restrict__
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
                     const float p = (q + k < 8) ? lo[q + k][c] : hi[q + k - 8][c];
-                    acc[q][c] = __builtin_fmaf(p, wk, acc[q][c]);
+                    acc[q][c] = mul_add_exact(acc[q][c], p, wk);
                 }
             }
         }
@@ -208,39 +237,87 @@ __global__ __launch_bounds__(kThreads) void k_blur_v(const BlurJob* __restrict__
 #pragma unroll
             for (int c = 0; c < 4; c++) lo[i][c] = hi[i][c];
     }
-
-    const float eps = J.eps_rel;
+    // remaining taps (< 8): lo holds rows yb - r + b .. +7
+    for (; b < n; b++) {
+        const float wk = taps[b];
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[q][c] = mul_add_exact(acc[q][c], lo[q][c], wk);
+#pragma unroll
+        for (int i = 0; i < 7; i++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) lo[i][c] = lo[i + 1][c];
+        int yy = min(max(yb - r + b + 8, 0), h - 1);
+        cvt4(*(const unsigned*)(col + (size_t)yy * pitch), lo[7]);
+    }
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         const int y = yb + q;
         if (y >= h) break;
-        unsigned res[4];
+        store_row4(J, y, x, acc[q]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Vertical pass, narrow octaves (and wide kernels). Tile = 64 columns x 16 rows;
+// lane = column, wave w owns rows y0+4w..+3. The tile's input rows
+// [y0-r, y0+16+r), clamped, are staged once in LDS as u8 (at most min(h, 16+2r)
+// distinct rows, so the LDS footprint stays small even when r >> h).
+// ---------------------------------------------------------------------------
+constexpr int kVnTileW = 64, kVnTileH = 16;
+
+__global__ __launch_bounds__(kThreads) void k_blur_v_narrow(const BlurJob* __restrict__ jobs,
+                                                            int njobs) {
+    extern __shared__ uint8_t vl[];
+    const int tile = blockIdx.x;
+    const int j = find_job(jobs, njobs, tile);
+    const BlurJob& J = jobs[j];
+    const int t = tile - J.tile_begin;
+    const int tx = t % J.tiles_x, ty = t / J.tiles_x;
+    const int w = J.w, h = J.h;
+    const int r = J.ntaps >> 1;
+    const int n = J.ntaps;
+    const int x0 = tx * kVnTileW, y0 = ty * kVnTileH;
+    const int rlo = max(y0 - r, 0), rhi = min(y0 + kVnTileH + r, h);  // distinct rows [rlo, rhi)
+    const int nrows = rhi - rlo;
+    for (int i = threadIdx.x; i < nrows * kVnTileW; i += kThreads) {
+        const int rr = i / kVnTileW, c = i - rr * kVnTileW;
+        const int x = min(x0 + c, w - 1);
+        vl[i] = J.src[(size_t)(rlo + rr) * J.src_pitch + x];
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int x = x0 + lane;
+    const int yb = y0 + wave * 4;
+    if (x >= w || yb >= h) return;
+    const float* __restrict__ taps = J.taps;
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float cur[4];
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            if (!fast_u8(acc[q][c], eps, res[c])) {
-                float s = 0.0f;
-                for (int i = 0; i < J.ntaps; i++) {
-                    int yy = min(max(y - r + i, 0), h - 1);
-                    s = mul_add_exact(s, (float)col[(size_t)yy * pitch + c], taps[i]);
-                }
-                res[c] = u8_trunc(s);
-            }
-        }
-        uint8_t* dst = J.dst + (size_t)y * J.dst_pitch + x;
-        if (x + 4 <= w) {
-            *(unsigned*)dst = res[0] | (res[1] << 8) | (res[2] << 16) | (res[3] << 24);
-        } else {
+    for (int q = 0; q < 4; q++) {
+        const int yy = min(max(yb + q - r, 0), h - 1);
+        cur[q] = (float)vl[(yy - rlo) * kVnTileW + lane];
+    }
+    for (int i = 0; i < n; i++) {
+        const float wk = taps[i];
 #pragma unroll
-            for (int c = 0; c < 4; c++)
-                if (x + c < w) dst[c] = (uint8_t)res[c];
-        }
-        if (J.dog) {
-            const uint8_t* prev = J.prev + (size_t)y * J.dst_pitch + x;
-            float* dg = J.dog + (size_t)y * J.dog_pitch + x;
+        for (int q = 0; q < 4; q++) acc[q] = mul_add_exact(acc[q], cur[q], wk);
 #pragma unroll
-            for (int c = 0; c < 4; c++)
-                if (x + c < w) dg[c] = q255(res[c]) - q255(prev[c]);
+        for (int q = 0; q < 3; q++) cur[q] = cur[q + 1];
+        if (i + 1 < n) {  // next input row; stays inside the staged rows
+            const int yy = min(max(yb + 4 + i - r, 0), h - 1);
+            cur[3] = (float)vl[(yy - rlo) * kVnTileW + lane];
         }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int y = yb + q;
+        if (y >= h) break;
+        const unsigned v = u8_trunc(acc[q]);
+        J.dst[(size_t)y * J.dst_pitch + x] = (uint8_t)v;
+        if (J.dog)
+            J.dog[(size_t)y * J.dog_pitch + x] = q255(v) - q255(J.prev[(size_t)y * J.dst_pitch + x]);
     }
 }
 
@@ -297,7 +374,10 @@ __global__ __launch_bounds__(kThreads) void k_copy_level(const BlurJob* __restri
 // ---------------------------------------------------------------------------
 // Lanczos3 resize (image 0.25.6 sample::resize): vertical pass into f32 rows
 // (t += p * w in window order, unrounded), then horizontal pass with
-// round(clamp(t, 0, 255)). Exact reference order (no FMA).
+// round(clamp(t, 0, 255)). Exact reference order (separate mul and add).
+// Vertical: a thread computes 4 adjacent columns of one output row (one u32
+// load per tap). Horizontal: a workgroup computes 256 outputs of one row from
+// the input span they need, staged in LDS.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void k_resize_v(const ResizeJob* __restrict__ jobs,
                                                        int njobs) {
@@ -305,44 +385,103 @@ __global__ __launch_bounds__(kThreads) void k_resize_v(const ResizeJob* __restri
     const ResizeJob& J = jobs[find_job(jobs, njobs, tile)];
     const int t = tile - J.tile_begin;
     const int oy = t / J.tiles_x;
-    const int x = (t % J.tiles_x) * kThreads + threadIdx.x;
+    const int x = ((t % J.tiles_x) * kThreads + threadIdx.x) * 4;
     if (x >= J.in_w) return;
     const int l = J.left[oy], n = J.count[oy];
     const float* wr = J.weights + (size_t)oy * J.maxtaps;
     const uint8_t* src = J.src_u8 + (size_t)l * J.src_pitch + x;
-    float s = 0.0f;
-    for (int i = 0; i < n; i++) s = mul_add_exact(s, (float)src[(size_t)i * J.src_pitch], wr[i]);
-    J.tmp[(size_t)oy * J.tmp_pitch + x] = s;
+    float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    int i = 0;
+    for (; i + 4 <= n; i += 4) {
+        unsigned u[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) u[k] = *(const unsigned*)(src + (size_t)(i + k) * J.src_pitch);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float wk = wr[i + k];
+            float p[4];
+            cvt4(u[k], p);
+#pragma unroll
+            for (int c = 0; c < 4; c++) s[c] = mul_add_exact(s[c], p[c], wk);
+        }
+    }
+    for (; i < n; i++) {
+        const float wk = wr[i];
+        float p[4];
+        cvt4(*(const unsigned*)(src + (size_t)i * J.src_pitch), p);
+#pragma unroll
+        for (int c = 0; c < 4; c++) s[c] = mul_add_exact(s[c], p[c], wk);
+    }
+    float* out = J.tmp + (size_t)oy * J.tmp_pitch + x;
+    if (x + 4 <= J.in_w) {
+        *(float4*)out = make_float4(s[0], s[1], s[2], s[3]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if (x + c < J.in_w) out[c] = s[c];
+    }
 }
+
+constexpr int kResizeSpan = 2 * kThreads + 64;  // input floats for 256 outputs (ratio ~2)
 
 __global__ __launch_bounds__(kThreads) void k_resize_h(const ResizeJob* __restrict__ jobs,
                                                        int njobs) {
+    __shared__ float span[kResizeSpan];
     const int tile = blockIdx.x;
     const ResizeJob& J = jobs[find_job(jobs, njobs, tile)];
     const int t = tile - J.tile_begin;
     const int y = t / J.tiles_x;
-    const int ox = (t % J.tiles_x) * kThreads + threadIdx.x;
+    const int ox0 = (t % J.tiles_x) * kThreads;
+    const int ox = ox0 + threadIdx.x;
+    const int oxl = min(ox0 + kThreads, J.out_w) - 1;
+    const float* src = J.tmp + (size_t)y * J.tmp_pitch;
+    const int lo = J.left[ox0];
+    const int hi = J.left[oxl] + J.count[oxl];  // exclusive
+    const bool staged = hi - lo <= kResizeSpan;
+    if (staged)
+        for (int i = threadIdx.x; i < hi - lo; i += kThreads) span[i] = src[lo + i];
+    __syncthreads();
     if (ox >= J.out_w) return;
     const int l = J.left[ox], n = J.count[ox];
     const float* wr = J.weights + (size_t)ox * J.maxtaps;
-    const float* src = J.tmp + (size_t)y * J.tmp_pitch + l;
+    const float* in = staged ? span + (l - lo) : src + l;
     float s = 0.0f;
-    for (int i = 0; i < n; i++) s = mul_add_exact(s, src[i], wr[i]);
+    if (n <= 16) {
+        float wv[16];  // issue all weight loads before the dependent sum
+#pragma unroll
+        for (int i = 0; i < 16; i++) wv[i] = i < n ? wr[i] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            if (i < n) s = mul_add_exact(s, in[i], wv[i]);
+    } else {
+        for (int i = 0; i < n; i++) s = mul_add_exact(s, in[i], wr[i]);
+    }
     float c = s < 0.0f ? 0.0f : (s > 255.0f ? 255.0f : s);
     J.dst[(size_t)y * J.dst_pitch + ox] = (uint8_t)roundf(c);
 }
 
 }  // namespace
 
-void launch_blur_h(const BlurJob* d_jobs, int njobs, int total_tiles, int max_pad, hipStream_t s) {
+void launch_blur_h(const BlurJob* d_jobs, int njobs, int total_tiles, int max_pad, int px,
+                   hipStream_t s) {
     if (total_tiles <= 0) return;
-    size_t lds = (size_t)kHTileH * (kHTileW + max_pad + 8) * sizeof(float);
-    hipLaunchKernelGGL(k_blur_h, dim3(total_tiles), dim3(kThreads), lds, s, d_jobs, njobs);
+    const int rowlen = 64 * px + max_pad + 8;
+    const size_t lds = (size_t)kHTileH * (hpad(rowlen) + 1) * sizeof(float);
+    if (px == 8)
+        hipLaunchKernelGGL(k_blur_h<8>, dim3(total_tiles), dim3(kThreads), lds, s, d_jobs, njobs);
+    else
+        hipLaunchKernelGGL(k_blur_h<2>, dim3(total_tiles), dim3(kThreads), lds, s, d_jobs, njobs);
 }
 
-void launch_blur_v(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s) {
+void launch_blur_v(const BlurJob* d_jobs, int njobs, int total_tiles, int narrow_rows,
+                   hipStream_t s) {
     if (total_tiles <= 0) return;
-    hipLaunchKernelGGL(k_blur_v, dim3(total_tiles), dim3(kThreads), 0, s, d_jobs, njobs);
+    if (narrow_rows > 0) {
+        size_t lds = (size_t)narrow_rows * kVnTileW;
+        hipLaunchKernelGGL(k_blur_v_narrow, dim3(total_tiles), dim3(kThreads), lds, s, d_jobs, njobs);
+    } else {
+        hipLaunchKernelGGL(k_blur_v, dim3(total_tiles), dim3(kThreads), 0, s, d_jobs, njobs);
+    }
 }
 
 void launch_blur_h_generic(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s) {

@@ -25,7 +25,6 @@ struct BlurJob {
     const float* taps;    // ntaps_pad floats, zero-padded
     int ntaps;            // 2r+1
     int ntaps_pad;        // multiple of kTapBlock
-    float eps_rel;        // fast-path error bound factor for ntaps taps
     int w, h;
     int src_pitch, dst_pitch, dog_pitch;
     int tiles_x;          // tiles along x for this job
@@ -86,8 +85,12 @@ struct Keypoint {
 };
 
 // Launch wrappers (kernels_pyramid.hip / kernels_detect.hip).
-void launch_blur_h(const BlurJob* d_jobs, int njobs, int total_tiles, int max_pad, hipStream_t s);
-void launch_blur_v(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
+// px: outputs per lane (8 for wide rows, 2 for narrow rows).
+void launch_blur_h(const BlurJob* d_jobs, int njobs, int total_tiles, int max_pad, int px,
+                   hipStream_t s);
+// narrow_rows > 0 selects the 64x16-tile LDS variant with that many staged rows.
+void launch_blur_v(const BlurJob* d_jobs, int njobs, int total_tiles, int narrow_rows,
+                   hipStream_t s);
 void launch_resize_v(const ResizeJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
 void launch_resize_h(const ResizeJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
 void launch_copy_level(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
@@ -96,6 +99,12 @@ void launch_blur_v_generic(const BlurJob* d_jobs, int njobs, int total_tiles, hi
 
 void launch_extrema(const OctaveDesc* d_oct, int noct, int total_rows, const ImageParams* d_params,
                     unsigned long long* d_bits, unsigned* d_counts, hipStream_t s);
+// Tiled extremum scan (64x16 tiles) for S in [1, 6]: zeroes the row-slot
+// counts, then one workgroup per tile; OctaveDesc.row_begin holds the tile prefix.
+bool extrema_tile_supported(int S);
+void launch_extrema_tile(int S, const OctaveDesc* d_oct, int noct, int total_tiles,
+                         const ImageParams* d_params, unsigned long long* d_bits, unsigned* d_counts,
+                         int nslots, hipStream_t s);
 void launch_scan_dev(const unsigned* d_in, int n_max, const unsigned* n_dev, unsigned* d_out,
                      unsigned* d_tmp, hipStream_t s);
 void launch_compact(const OctaveDesc* d_oct, int noct, int nslots, const unsigned long long* d_bits,
