@@ -35,7 +35,109 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget for the CPU baseline sample (0 disables it)")
     ap.add_argument("--no-profile", action="store_true", help="skip the HIP-event roofline")
+    ap.add_argument("--mixed-batch", type=int, default=64,
+                    help="frames in the mixed-size batch leg (0 disables it)")
+    ap.add_argument("--serve-seconds", type=float, default=5.0,
+                    help="length of the served open-loop stream (0 disables it)")
+    ap.add_argument("--serve-rate", type=float, default=1000.0, help="requests/s of the stream")
+    ap.add_argument("--traffic", default=os.path.join(REPO, "profiles", "traffic_1080p_b32.txt"),
+                    help="rocprofv3 FETCH/WRITE summary giving HBM bytes per launch")
     return ap.parse_args()
+
+
+def mixed_batch(n, step):
+    """n frames with sizes drawn from 640x480 / 1280x720 / 1920x1080 / 2048x1536,
+    frame i seed 1000+i, octaves floor(log2(min(w,h)))-3; one batched call vs n
+    per-image calls, inputs resident in HBM."""
+    import torch
+    import imgen
+    import sift_hip
+    shapes = imgen.batch_shapes(n)
+    imgs = [imgen.generic_image(w, h, seed=1000 + i) for i, (w, h) in enumerate(shapes)]
+    dev = [torch.from_numpy(a).to("cuda") for a in imgs]
+    torch.cuda.synchronize()
+    params = [sift_hip.Sift(1.6, imgen.octaves_for(w, h), 3, 0.5, 0.04, 10.0) for (w, h) in shapes]
+    items = [(t.data_ptr(), w, h, w) for t, (w, h) in zip(dev, shapes)]
+    s = sift_hip.Sift.default()
+    for _ in range(2):
+        s.detect_batch_device(items, params)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        counts, _, _, _ = s.detect_batch_device(items, params)
+    tb = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for it, p in zip(items, params):
+        p.detect_batch_device([it])
+    tp = time.perf_counter() - t0
+    mpix = sum(w * h for w, h in shapes) / 1e6
+    return {"frames": n, "sizes": {f"{w}x{h}": shapes.count((w, h)) for (w, h) in imgen.BATCH_SHAPES},
+            "batched_frames_per_s": round(n / tb, 2), "batched_mpix_per_s": round(mpix / tb, 1),
+            "batch_ms": round(tb * 1e3, 3),
+            "per_image_frames_per_s": round(n / tp, 2), "per_image_mpix_per_s": round(mpix / tp, 1),
+            "batched_vs_per_image": round(tp / tb, 3),
+            "keypoints": int(sum(counts))}
+
+
+def served(seconds, rate, seed=7):
+    """Open-loop Poisson stream: sizes 40/30/20/10% of 640x480 / 1280x720 /
+    1920x1080 / 3840x2160 (octaves floor(log2(min))-3), host frames in, host
+    results out; latency = completion - submit."""
+    import threading
+    import numpy as np
+    import imgen
+    import sift_hip
+    sizes = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+    mix = [0.4, 0.3, 0.2, 0.1]
+    pool = {sz: [imgen.generic_image(sz[0], sz[1], seed=100000 + 10 * k + i) for i in range(4)]
+            for k, sz in enumerate(sizes)}
+    rng = np.random.default_rng(seed)
+    n = int(seconds * rate)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    which = rng.choice(4, size=n, p=mix)
+    per_size = {sz: sift_hip.Sift(1.6, imgen.octaves_for(*sz), 3, 0.5, 0.04, 10.0) for sz in sizes}
+    sift = sift_hip.Sift.default()
+    lat = {sz: [] for sz in sizes}
+    done = []
+    with sift_hip.SiftServer(sift, hbm_budget_bytes=32 << 30, max_batch=64) as srv:
+        # warm-up: one frame of each size
+        for k, sz in enumerate(sizes):
+            srv.submit(pool[sz][0], 10**9 + k, per_size[sz])
+        got = 0
+        while got < 4:
+            got += len(srv.poll(timeout_ms=1000))
+        t_start = time.perf_counter()
+
+        def producer():
+            for j in range(n):
+                delay = t_start + arrivals[j] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                sz = sizes[which[j]]
+                srv.submit(pool[sz][j % 4], j, per_size[sz])
+        th = threading.Thread(target=producer)
+        th.start()
+        while len(done) < n:
+            for tag, st, kps, desc, ts, t0, t1 in srv.poll(timeout_ms=500):
+                if tag >= 10**9:
+                    continue
+                done.append((tag, st, t1 - ts))
+                lat[sizes[which[tag]]].append(t1 - ts)
+            if time.perf_counter() - t_start > seconds + 120:
+                break
+        th.join()
+        wall = time.perf_counter() - t_start
+    all_lat = np.array([d[2] for d in done])
+    q = lambda a, p: round(float(np.percentile(a, p)), 3) if len(a) else None
+    mp = sum(sizes[which[t]][0] * sizes[which[t]][1] for t, _, _ in done) / 1e6
+    return {"requests": n, "completed": len(done), "failed": sum(1 for d in done if d[1] != 0),
+            "rate_req_per_s": rate, "seconds": seconds, "frames_per_s": round(len(done) / wall, 1),
+            "mpix_per_s": round(mp / wall, 1),
+            "p50_ms": q(all_lat, 50), "p99_ms": q(all_lat, 99), "p99_9_ms": q(all_lat, 99.9),
+            "by_size": {f"{w}x{h}": {"n": len(lat[(w, h)]), "p50_ms": q(lat[(w, h)], 50),
+                                     "p99_ms": q(lat[(w, h)], 99)} for (w, h) in sizes},
+            "octaves": {f"{w}x{h}": imgen.octaves_for(w, h) for (w, h) in sizes},
+            "io": "host frames in, host results out (PCIe included)"}
 
 
 def frames(n):
@@ -150,6 +252,20 @@ def main():
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                            "traffic": None, "bytes_per_launch": round(by / n),
                            "avg_launch_us": round(avg_s * 1e6, 3)}
+        if os.path.exists(args.traffic):
+            tr = {}
+            for line in open(args.traffic):
+                parts = line.split()
+                if len(parts) == 3 and parts[0].startswith("k_"):
+                    tr[parts[0]] = (float(parts[1]), float(parts[2]))
+            if dom in tr:
+                out["roofline"]["traffic"] = round(tr[dom][0] + tr[dom][1])
+                out["roofline"]["traffic_read_write"] = [round(tr[dom][0]), round(tr[dom][1])]
+        out["dominant_kernel"] = max(stats.items(), key=lambda kv: kv[1][1])[0]
+    if args.mixed_batch > 0:
+        out["mixed_batch"] = mixed_batch(args.mixed_batch, step)
+    if args.serve_seconds > 0:
+        out["served"] = served(args.serve_seconds, args.serve_rate)
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(imgs[0], octv, ct, args.cpu_seconds)
     print(json.dumps(out))

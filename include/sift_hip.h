@@ -172,8 +172,12 @@ typedef struct sift_completion_t {
 } sift_completion_t;
 
 int sift_hip_server_start(sift_hip_ctx* ctx, const sift_server_config_t* cfg);
-/* Copies the pixels before returning. */
-int sift_hip_submit(sift_hip_ctx* ctx, const sift_image_t* image, uint64_t tag);
+/* Copies the pixels before returning. params: NULL = the context's params,
+ * else this frame's (e.g. an octave count chosen by frame size).
+ * SIFT_ERR_BUSY when the queue is full, SIFT_ERR_NOMEM when the frame alone
+ * exceeds the HBM budget. */
+int sift_hip_submit(sift_hip_ctx* ctx, const sift_image_t* image, const sift_params_t* params,
+                    uint64_t tag);
 int sift_hip_poll(sift_hip_ctx* ctx, sift_completion_t* out, size_t cap, size_t* n,
                   int timeout_ms);
 int sift_hip_server_stop(sift_hip_ctx* ctx);

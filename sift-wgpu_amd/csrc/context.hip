@@ -417,7 +417,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     const int noct = (int)octs.size();
 
     // capacities (grow-and-retry on overflow)
-    size_t want_cap = std::max<size_t>(4096, pix_layers / 48);
+    size_t want_cap = std::max<size_t>(4096, pix_layers / 128);
     if (cand_cap < want_cap) cand_cap = want_cap;
     if (kp_cap < cand_cap + cand_cap / 2) kp_cap = cand_cap + cand_cap / 2;
     const int cap = (int)cand_cap, kcap = (int)kp_cap;
@@ -835,6 +835,25 @@ int collect(sh::Context* C, int n, sh::RunOutputs& R, sift_result_t* out, bool* 
 
 }  // namespace
 
+namespace sh {
+// Runs a host-input batch and copies the results out, growing the candidate
+// and keypoint capacities and re-running when a count overflowed them.
+int run_and_collect(Context* C, const Batch& B, sift_result_t* out) {
+    const int n = (int)B.images.size();
+    for (int i = 0; i < n; i++) out[i] = sift_result_t{nullptr, nullptr, 0};
+    for (int attempt = 0; attempt < 4; attempt++) {
+        RunOutputs R;
+        int rc = C->run(B, &R);
+        if (rc) return rc;
+        bool retry = false;
+        rc = collect(C, n, R, out, &retry);
+        if (rc) return rc;
+        if (!retry) return SIFT_OK;
+    }
+    return fail(SIFT_ERR_NOMEM, "keypoint capacity did not converge");
+}
+}  // namespace sh
+
 extern "C" {
 
 void sift_params_default(sift_params_t* out) {
@@ -910,16 +929,7 @@ int sift_hip_detect_batch(sift_hip_ctx* ctx, const sift_image_t* images, const s
     B.images.assign(images, images + n);
     B.params.assign(n, C->params);
     if (params) B.params.assign(params, params + n);
-    for (int attempt = 0; attempt < 4; attempt++) {
-        sh::RunOutputs R;
-        int rc = C->run(B, &R);
-        if (rc) return rc;
-        bool retry = false;
-        rc = collect(C, (int)n, R, out, &retry);
-        if (rc) return rc;
-        if (!retry) return SIFT_OK;
-    }
-    return fail(SIFT_ERR_NOMEM, "keypoint capacity did not converge");
+    return sh::run_and_collect(C, B, out);
 }
 
 int sift_hip_detect(sift_hip_ctx* ctx, const sift_image_t* image, sift_result_t* out) {

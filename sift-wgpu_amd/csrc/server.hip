@@ -1,17 +1,253 @@
-// server.hip -- continuous-batching request loop (placeholder until the
-// serving path lands; the entry points report SIFT_ERR_INVALID).
+// server.hip -- continuous-batching request loop on one device.
+//
+// sift_hip_submit copies a frame into host memory and queues it. Two worker
+// lanes, each with its own HIP stream and device arena, take every queued frame
+// that fits (up to max_batch, and while the bytes of admitted frames stay within
+// the HBM budget), run them as one batch through the same kernels as
+// sift_hip_detect_batch, and post completions. Frames that arrive while a batch
+// is on the device go into the next batch of whichever lane frees first, so two
+// batches overlap on the device and queueing stays short under load.
+#include <string.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
 #include "sift_context.h"
 
 namespace sh {
-struct Server {};
+
+namespace {
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+size_t round_up_sz(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace
+
+// Device bytes one frame needs in a batch (pyramid, temporaries, detection
+// buffers at the default capacity), used for admission against the budget.
+size_t frame_device_bytes(uint32_t w, uint32_t h, const sift_params_t& p) {
+    const size_t L = p.num_intervals + 3;
+    size_t bytes = round_up_sz(w, 64) * h;  // input copy
+    size_t px_layers = 0;
+    uint32_t ww = w, hh = h;
+    for (uint32_t o = 0; o < p.num_octaves; o++) {
+        bytes += L * round_up_sz(ww ? ww : 1, 64) * hh + 256;
+        px_layers += (size_t)p.num_intervals * ww * hh;
+        ww /= 2;
+        hh /= 2;
+    }
+    bytes += round_up_sz(w, 64) * h * 3;  // blur and resize temporaries
+    const size_t cand = std::max<size_t>(4096, px_layers / 128);
+    bytes += cand * (sizeof(Cand) + 4 + sizeof(Keypoint) + 4 + 4 + kMaxPeaks * 4 + 4);
+    bytes += cand * 3 / 2 * (sizeof(Keypoint) + 128 * 4);
+    return bytes;
+}
+
+int run_and_collect(Context* C, const Batch& B, sift_result_t* out);
+
+struct Request {
+    uint64_t tag;
+    uint32_t w, h;
+    sift_params_t params;
+    std::vector<uint8_t> pixels;
+    double submit_ms;
+    size_t bytes;
+};
+
+struct Server {
+    Context* owner = nullptr;
+    sift_server_config_t cfg{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request> pending;
+    std::deque<sift_completion_t> done;
+    size_t bytes_in_flight = 0;
+    bool stopping = false;
+    std::vector<Context*> lanes;
+    std::vector<std::thread> workers;
+
+    void worker(Context* lane);
+};
+
+void Server::worker(Context* lane) {
+    (void)hipSetDevice(lane->device);
+    for (;;) {
+        std::vector<Request> batch;
+        size_t batch_bytes = 0;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            for (;;) {
+                if (stopping && pending.empty()) return;
+                while (!pending.empty() && batch.size() < cfg.max_batch) {
+                    Request& r = pending.front();
+                    if (bytes_in_flight + r.bytes > cfg.hbm_budget_bytes) break;
+                    bytes_in_flight += r.bytes;
+                    batch_bytes += r.bytes;
+                    batch.push_back(std::move(r));
+                    pending.pop_front();
+                }
+                if (!batch.empty()) break;
+                cv_work.wait(lk);  // woken by submit, by memory release, or by stop
+            }
+        }
+        const double start = now_ms();
+        Batch B;
+        B.images.resize(batch.size());
+        B.params.resize(batch.size());
+        for (size_t i = 0; i < batch.size(); i++) {
+            B.images[i] = sift_image_t{batch[i].pixels.data(), batch[i].w, batch[i].h, batch[i].w};
+            B.params[i] = batch[i].params;
+        }
+        std::vector<sift_result_t> res(batch.size());
+        int rc = run_and_collect(lane, B, res.data());
+        const double end = now_ms();
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (size_t i = 0; i < batch.size(); i++) {
+                sift_completion_t c{};
+                c.tag = batch[i].tag;
+                c.status = rc;
+                if (rc == SIFT_OK) c.result = res[i];
+                c.submit_ms = batch[i].submit_ms;
+                c.start_ms = start;
+                c.done_ms = end;
+                done.push_back(c);
+            }
+            bytes_in_flight -= batch_bytes;
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+}
+
 }  // namespace sh
 
+using namespace sh;
+
 extern "C" {
-int sift_hip_server_start(sift_hip_ctx*, const sift_server_config_t*) { return SIFT_ERR_INVALID; }
-int sift_hip_submit(sift_hip_ctx*, const sift_image_t*, uint64_t) { return SIFT_ERR_INVALID; }
-int sift_hip_poll(sift_hip_ctx*, sift_completion_t*, size_t, size_t* n, int) {
+
+int sift_hip_server_start(sift_hip_ctx* ctx, const sift_server_config_t* cfg) {
+    if (!ctx || !cfg) return SIFT_ERR_INVALID;
+    Context* C = reinterpret_cast<Context*>(ctx);
+    if (C->server) return SIFT_ERR_INVALID;
+    Server* S = new (std::nothrow) Server();
+    if (!S) return SIFT_ERR_NOMEM;
+    S->owner = C;
+    S->cfg = *cfg;
+    if (S->cfg.max_batch == 0) S->cfg.max_batch = 64;
+    if (S->cfg.max_queue == 0) S->cfg.max_queue = 4096;
+    if (S->cfg.hbm_budget_bytes == 0) S->cfg.hbm_budget_bytes = (size_t)32 << 30;
+    for (int i = 0; i < 2; i++) {
+        Context* L = new (std::nothrow) Context();
+        if (!L) break;
+        L->device = C->device;
+        L->params = C->params;
+        if (hipSetDevice(C->device) != hipSuccess ||
+            hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete L;
+            break;
+        }
+        S->lanes.push_back(L);
+    }
+    if (S->lanes.empty()) {
+        delete S;
+        return SIFT_ERR_DEVICE;
+    }
+    for (Context* L : S->lanes) S->workers.emplace_back([S, L] { S->worker(L); });
+    C->server = S;
+    return SIFT_OK;
+}
+
+int sift_hip_submit(sift_hip_ctx* ctx, const sift_image_t* image, const sift_params_t* params,
+                    uint64_t tag) {
+    const double t_submit = now_ms();
+    if (!ctx || !image || !image->data || image->width == 0 || image->height == 0 ||
+        image->width > 65536 || image->height > 65536 ||
+        (image->stride && image->stride < image->width))
+        return SIFT_ERR_INVALID;
+    Context* C = reinterpret_cast<Context*>(ctx);
+    Server* S = C->server;
+    if (!S) return SIFT_ERR_INVALID;
+    const sift_params_t& P = params ? *params : C->params;
+    if (P.num_intervals < 1 || P.num_intervals > 13 || P.num_octaves < 1 || P.num_octaves > 16 ||
+        !(P.sigma > 0.0f))
+        return SIFT_ERR_INVALID;
+    Request r;
+    r.tag = tag;
+    r.w = image->width;
+    r.h = image->height;
+    r.params = P;
+    r.bytes = frame_device_bytes(r.w, r.h, P);
+    if (r.bytes > S->cfg.hbm_budget_bytes) return SIFT_ERR_NOMEM;
+    r.pixels.resize((size_t)r.w * r.h);
+    const size_t stride = image->stride ? image->stride : image->width;
+    for (uint32_t y = 0; y < r.h; y++)
+        memcpy(r.pixels.data() + (size_t)y * r.w, image->data + (size_t)y * stride, r.w);
+    r.submit_ms = t_submit;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->stopping) return SIFT_ERR_INVALID;
+        if (S->pending.size() >= S->cfg.max_queue) return SIFT_ERR_BUSY;
+        S->pending.push_back(std::move(r));
+    }
+    S->cv_work.notify_one();
+    return SIFT_OK;
+}
+
+int sift_hip_poll(sift_hip_ctx* ctx, sift_completion_t* out, size_t cap, size_t* n, int timeout_ms) {
     if (n) *n = 0;
-    return SIFT_ERR_INVALID;
+    if (!ctx || !out || !n) return SIFT_ERR_INVALID;
+    Context* C = reinterpret_cast<Context*>(ctx);
+    Server* S = C->server;
+    if (!S) return SIFT_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (S->done.empty() && timeout_ms != 0) {
+        if (timeout_ms < 0) S->cv_done.wait(lk, [&] { return !S->done.empty(); });
+        else S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms),
+                                 [&] { return !S->done.empty(); });
+    }
+    size_t k = 0;
+    while (k < cap && !S->done.empty()) {
+        out[k++] = S->done.front();
+        S->done.pop_front();
+    }
+    *n = k;
+    return k ? SIFT_OK : SIFT_ERR_TIMEOUT;
 }
-int sift_hip_server_stop(sift_hip_ctx*) { return SIFT_OK; }
+
+int sift_hip_server_stop(sift_hip_ctx* ctx) {
+    if (!ctx) return SIFT_ERR_INVALID;
+    Context* C = reinterpret_cast<Context*>(ctx);
+    Server* S = C->server;
+    if (!S) return SIFT_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stopping = true;
+    }
+    S->cv_work.notify_all();
+    for (auto& t : S->workers) t.join();
+    for (auto& c : S->done) sift_result_free(&c.result);
+    for (Context* L : S->lanes) {
+        (void)hipSetDevice(L->device);
+        if (L->stream) (void)hipStreamSynchronize(L->stream);
+        if (L->arena) (void)hipFree(L->arena);
+        if (L->tables.dev) (void)hipFree(L->tables.dev);
+        if (L->pinned) (void)hipHostFree(L->pinned);
+        for (hipEvent_t e : L->prof_events) (void)hipEventDestroy(e);
+        if (L->stream) (void)hipStreamDestroy(L->stream);
+        delete L;
+    }
+    delete S;
+    C->server = nullptr;
+    return SIFT_OK;
 }
+
+}  // extern "C"

@@ -125,7 +125,7 @@ def lib():
         L.sift_hip_profile_read.restype = ctypes.c_int
         L.sift_hip_server_start.argtypes = [ctypes.c_void_p, P(SiftServerConfig)]
         L.sift_hip_server_start.restype = ctypes.c_int
-        L.sift_hip_submit.argtypes = [ctypes.c_void_p, P(SiftImage), ctypes.c_uint64]
+        L.sift_hip_submit.argtypes = [ctypes.c_void_p, P(SiftImage), P(SiftParams), ctypes.c_uint64]
         L.sift_hip_submit.restype = ctypes.c_int
         L.sift_hip_poll.argtypes = [ctypes.c_void_p, P(SiftCompletion), ctypes.c_size_t,
                                     P(ctypes.c_size_t), ctypes.c_int]
@@ -396,3 +396,58 @@ def profile_read(device=0):
     _check(lib().sift_hip_profile_read(ctx, arr, n.value, ctypes.byref(n)), "sift_hip_profile_read")
     return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms), float(arr[i].bytes))
             for i in range(n.value)}
+
+
+class SiftServer:
+    """Continuous-batching request loop (sift_hip_server_start / submit / poll).
+
+    submit() copies the frame and returns at once; poll() returns completions
+    as (tag, status, keypoints, descriptors, submit_ms, start_ms, done_ms)."""
+
+    def __init__(self, sift=None, hbm_budget_bytes=32 << 30, max_batch=64, max_queue=4096,
+                 device=0):
+        sift = sift or Sift.default()
+        p = sift.params()
+        self._ctx = ctypes.c_void_p()
+        _check(lib().sift_hip_create(ctypes.byref(p), device, ctypes.byref(self._ctx)),
+               "sift_hip_create")
+        cfg = SiftServerConfig(hbm_budget_bytes, max_batch, max_queue)
+        rc = lib().sift_hip_server_start(self._ctx, ctypes.byref(cfg))
+        if rc != SIFT_OK:
+            lib().sift_hip_destroy(self._ctx)
+            raise SiftError(f"sift_hip_server_start failed ({rc})")
+        self._keep = {}
+
+    def submit(self, img, tag, sift=None):
+        """Queues a frame (copied before returning); `sift` optionally gives this
+        frame's parameters (a Sift), else the server's."""
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        h, w = img.shape
+        s = SiftImage(img.ctypes.data, w, h, w)
+        pp = ctypes.byref(sift.params()) if sift is not None else None
+        rc = lib().sift_hip_submit(self._ctx, ctypes.byref(s), pp, tag)
+        if rc != SIFT_OK:
+            raise SiftError(f"sift_hip_submit failed ({rc})")
+
+    def poll(self, cap=256, timeout_ms=100):
+        arr = (SiftCompletion * cap)()
+        n = ctypes.c_size_t()
+        lib().sift_hip_poll(self._ctx, arr, cap, ctypes.byref(n), timeout_ms)
+        out = []
+        for i in range(n.value):
+            c = arr[i]
+            kps, desc = _take_result(c.result) if c.status == SIFT_OK else (None, None)
+            out.append((int(c.tag), int(c.status), kps, desc, c.submit_ms, c.start_ms, c.done_ms))
+        return out
+
+    def close(self):
+        if self._ctx:
+            lib().sift_hip_server_stop(self._ctx)
+            lib().sift_hip_destroy(self._ctx)
+            self._ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
