@@ -88,22 +88,33 @@ namespace sh {
 // ---------------------------------------------------------------------------
 // Table store: taps and Lanczos tables in one growing device buffer.
 // ---------------------------------------------------------------------------
-int TableStore::ensure_uploaded() {
+int TableStore::ensure_uploaded(hipStream_t stream) {
     if (host.size() == uploaded) return SIFT_OK;
     if (host.size() > dev_cap) {
-        size_t ncap = std::max(host.size() * 2, (size_t)1 << 20);
+        // rare: the first 16 MiB hold the tables of many parameter sets and sizes
+        size_t ncap = std::max(host.size() * 2, (size_t)16 << 20);
         void* nd = nullptr;
         HIP_TRY(hipMalloc(&nd, ncap));
         if (dev) {
-            (void)hipDeviceSynchronize();
+            HIP_TRY(hipStreamSynchronize(stream));
             (void)hipFree(dev);
         }
         dev = (char*)nd;
         dev_cap = ncap;
         uploaded = 0;
     }
-    HIP_TRY(hipMemcpy(dev + uploaded, host.data() + uploaded, host.size() - uploaded,
-                      hipMemcpyHostToDevice));
+    const size_t n = host.size() - uploaded;
+    if (n > bounce_cap) {
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (bounce) (void)hipHostFree(bounce);
+        bounce = nullptr;
+        bounce_cap = 0;
+        size_t ncap = std::max(n * 2, (size_t)1 << 20);
+        HIP_TRY(hipHostMalloc((void**)&bounce, ncap, hipHostMallocDefault));
+        bounce_cap = ncap;
+    }
+    memcpy(bounce, host.data() + uploaded, n);
+    HIP_TRY(hipMemcpyAsync(dev + uploaded, bounce, n, hipMemcpyHostToDevice, stream));
     uploaded = host.size();
     return SIFT_OK;
 }
@@ -214,9 +225,48 @@ int Context::ensure_arena(size_t bytes) {
     return SIFT_OK;
 }
 
+int Context::ensure_pinned_out(size_t bytes) {
+    if (bytes <= pinned_out_cap) return SIFT_OK;
+    // geometric growth: a stream of growing batches reallocates O(log) times
+    size_t ncap = std::max(std::max(bytes + bytes / 4, 2 * pinned_out_cap), (size_t)1 << 20);
+    if (pinned_out) {
+        HIP_TRY(hipStreamSynchronize(stream));
+        (void)hipHostFree(pinned_out);
+        pinned_out = nullptr;
+        pinned_out_cap = 0;
+    }
+    void* p = nullptr;
+    hipError_t e = hipHostMalloc(&p, ncap, hipHostMallocDefault);
+    if (e != hipSuccess) return fail(SIFT_ERR_NOMEM, "pinned host allocation failed");
+    pinned_out = (char*)p;
+    pinned_out_cap = ncap;
+    return SIFT_OK;
+}
+
+void Context::release() {
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    prof_resolve();
+    for (hipEvent_t e : prof_events) (void)hipEventDestroy(e);
+    prof_events.clear();
+    if (arena) (void)hipFree(arena);
+    if (tables.dev) (void)hipFree(tables.dev);
+    if (tables.bounce) (void)hipHostFree(tables.bounce);
+    if (pinned) (void)hipHostFree(pinned);
+    if (pinned_out) (void)hipHostFree(pinned_out);
+    if (stream) (void)hipStreamDestroy(stream);
+    arena = nullptr;
+    tables.dev = nullptr;
+    tables.bounce = nullptr;
+    pinned = pinned_out = nullptr;
+    stream = nullptr;
+    arena_cap = tables.dev_cap = tables.bounce_cap = pinned_cap = pinned_out_cap = 0;
+}
+
 int Context::ensure_pinned(size_t bytes) {
     if (bytes <= pinned_cap) return SIFT_OK;
-    size_t ncap = std::max(bytes + bytes / 4, (size_t)1 << 20);
+    // geometric growth: a stream of growing batches reallocates O(log) times
+    size_t ncap = std::max(std::max(bytes + bytes / 4, 2 * pinned_cap), (size_t)1 << 20);
     if (pinned) {
         HIP_TRY(hipStreamSynchronize(stream));
         (void)hipHostFree(pinned);
@@ -257,12 +307,13 @@ struct LaunchRec {
 enum KernelId {
     K_BLUR_H = 0, K_BLUR_V, K_COPY, K_BLUR_H_GENERIC, K_RESIZE_V, K_RESIZE_H, K_EXTREMA, K_SCAN,
     K_COMPACT, K_REFINE, K_ORIENT, K_DESCRIPTOR, K_FINALIZE, K_BLUR_V_NARROW, K_BLUR_V_GENERIC,
-    K_COUNT
+    K_BLUR_H2, K_COUNT
 };
+// Names as rocprofv3 reports the kernels (template arguments included).
 const char* const kKernelNames[K_COUNT] = {
-    "k_blur_h", "k_blur_v", "k_copy_level", "k_blur_h_generic", "k_resize_v", "k_resize_h",
-    "k_extrema", "k_scan", "k_compact", "k_refine", "k_orient", "k_descriptor", "k_finalize",
-    "k_blur_v_narrow", "k_blur_v_generic"};
+    "k_blur_h<8>", "k_blur_v", "k_copy_level", "k_blur_h_generic", "k_resize_v", "k_resize_h",
+    "k_extrema_tile", "k_scan", "k_compact", "k_refine", "k_orient", "k_descriptor",
+    "k_finalize", "k_blur_v_narrow", "k_blur_v_generic", "k_blur_h<2>"};
 
 double blur_job_bytes(const BlurJob& j, int kind) {
     double a = (double)j.w * j.h;
@@ -348,7 +399,9 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             }
         }
     }
-    int rc = tables.ensure_uploaded();
+    // the previous call synchronised the stream before returning, so the table
+    // bounce buffer and the pinned staging are free to reuse
+    int rc = tables.ensure_uploaded(stream);
     if (rc) return rc;
 
     // --- device layout ---
@@ -555,7 +608,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         for (auto& j : hj) maxpad = std::max(maxpad, j.ntaps_pad);
         for (auto& j : hj2) maxpad2 = std::max(maxpad2, j.ntaps_pad);
         push_launch(K_BLUR_H, hj, maxpad, 8);
-        push_launch(K_BLUR_H, hj2, maxpad2, 2);
+        push_launch(K_BLUR_H2, hj2, maxpad2, 2);
         push_launch(K_BLUR_H_GENERIC, hgj, 0, 0);
         push_launch(K_BLUR_V, vj, 0, 0);
         push_launch(K_BLUR_V_NARROW, vnj, 0, vn_rows);
@@ -625,7 +678,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             H.weights = (const float*)(T + LH.w_off);
             H.maxtaps = LH.maxtaps;
             H.tiles_x = (b.w + 255) / 256;
-            H.tiles = H.tiles_x * b.h;
+            H.tiles = H.tiles_x * ((b.h + 7) / 8);
             rh.push_back(H);
         }
         push_launch(K_RESIZE_V, rv, 0, 0);
@@ -679,6 +732,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         prof_begin(L.kind, L.bytes, &ev);
         switch (L.kind) {
             case K_BLUR_H:
+            case K_BLUR_H2:
                 launch_blur_h((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.max_pad, L.variant, st);
                 break;
             case K_BLUR_V: launch_blur_v((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, 0, st); break;
@@ -809,11 +863,14 @@ int collect(sh::Context* C, int n, sh::RunOutputs& R, sift_result_t* out, bool* 
         return SIFT_OK;
     }
     if (!out) return SIFT_OK;
-    std::vector<sh::Keypoint> kps(nkp);
-    std::vector<float> desc(nkp * 128);
+    const size_t kbytes = nkp * sizeof(sh::Keypoint), dbytes = nkp * 128 * 4;
+    int rc = C->ensure_pinned_out(kbytes + dbytes + 256);
+    if (rc) return rc;
+    const sh::Keypoint* kps = (const sh::Keypoint*)C->pinned_out;
+    const float* desc = (const float*)(C->pinned_out + ((kbytes + 255) / 256) * 256);
     if (nkp) {
-        HIP_TRY(hipMemcpyAsync(kps.data(), R.d_kps, nkp * sizeof(sh::Keypoint), hipMemcpyDeviceToHost, C->stream));
-        HIP_TRY(hipMemcpyAsync(desc.data(), R.d_desc, nkp * 128 * 4, hipMemcpyDeviceToHost, C->stream));
+        HIP_TRY(hipMemcpyAsync((void*)kps, R.d_kps, kbytes, hipMemcpyDeviceToHost, C->stream));
+        HIP_TRY(hipMemcpyAsync((void*)desc, R.d_desc, dbytes, hipMemcpyDeviceToHost, C->stream));
         HIP_TRY(hipStreamSynchronize(C->stream));
     }
     for (int i = 0; i < n; i++) {
@@ -826,8 +883,8 @@ int collect(sh::Context* C, int n, sh::RunOutputs& R, sift_result_t* out, bool* 
             out[i].keypoints = (sift_keypoint_t*)malloc(c * sizeof(sift_keypoint_t));
             out[i].descriptors = (float*)malloc(c * 128 * sizeof(float));
             if (!out[i].keypoints || !out[i].descriptors) return fail(SIFT_ERR_NOMEM, "host allocation failed");
-            memcpy(out[i].keypoints, kps.data() + b, c * sizeof(sift_keypoint_t));
-            memcpy(out[i].descriptors, desc.data() + b * 128, c * 128 * sizeof(float));
+            memcpy(out[i].keypoints, kps + b, c * sizeof(sift_keypoint_t));
+            memcpy(out[i].descriptors, desc + b * 128, c * 128 * sizeof(float));
         }
     }
     return SIFT_OK;
@@ -902,14 +959,7 @@ void sift_hip_destroy(sift_hip_ctx* ctx) {
     if (!ctx) return;
     sh::Context* C = reinterpret_cast<sh::Context*>(ctx);
     sift_hip_server_stop(ctx);
-    (void)hipSetDevice(C->device);
-    if (C->stream) (void)hipStreamSynchronize(C->stream);
-    C->prof_resolve();
-    for (hipEvent_t e : C->prof_events) (void)hipEventDestroy(e);
-    if (C->arena) (void)hipFree(C->arena);
-    if (C->tables.dev) (void)hipFree(C->tables.dev);
-    if (C->pinned) (void)hipHostFree(C->pinned);
-    if (C->stream) (void)hipStreamDestroy(C->stream);
+    C->release();
     delete C;
 }
 

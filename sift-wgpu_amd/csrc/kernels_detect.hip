@@ -743,11 +743,16 @@ __global__ __launch_bounds__(kThreads) void k_descriptor(
     const float* __restrict__ angles, const unsigned* __restrict__ koff, int kcap,
     Keypoint* __restrict__ kps_out, float* __restrict__ desc_out) {
     constexpr int kPatch = 2 * kDescMaxR + 6;
+    // The u8 patch is dead once the bilinear grid is built; the histograms are
+    // live only after that, so they share storage.
+    constexpr int kUnion = (kPatch * kPatch + 3) / 4 > 4 * kDescPadBins ? (kPatch * kPatch + 3) / 4
+                                                                         : 4 * kDescPadBins;
     __shared__ float q[256];
-    __shared__ uint8_t patch[kPatch * kPatch];
+    __shared__ unsigned scratch_u[kUnion];
     __shared__ float grid[kDescGrid];
     __shared__ float ex[kDescMaxR + 2];
-    __shared__ unsigned hist[4][kDescPadBins];
+    uint8_t* const patch = reinterpret_cast<uint8_t*>(scratch_u);
+    unsigned (*const hist)[kDescPadBins] = reinterpret_cast<unsigned (*)[kDescPadBins]>(scratch_u);
     fill_q(q);
     __syncthreads();
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -799,6 +804,7 @@ __global__ __launch_bounds__(kThreads) void k_descriptor(
                           q12 * (1.0f - dx) * dy + q22 * dx * dy;
             }
         }
+        __syncthreads();  // patch reads done before the histograms reuse its storage
         const int per = (nsamp + kThreads - 1) / kThreads;
         const int chunk0 = (lane * 4 + wave) * per;  // lanes of a wave spread over the window
         // Gaussian weight exp(-(px^2+py^2)/denom) as exp(-px^2/denom) * exp(-py^2/denom)
@@ -838,6 +844,7 @@ __global__ __launch_bounds__(kThreads) void k_descriptor(
                         const float sy = sa * py, cy_ = ca * py;  // per-row terms of rx, ry
                         const float wy = ex[abs(cy - R)];
                         const float* gr = grid + (cy + 1) * gside + (oxx + kx0 + 1);
+#pragma unroll 2
                         for (int kx = kx0; kx <= kx1; kx++, gr++) {
                             const int cx = oxx + kx;
                             const float px = (float)(cx - R);

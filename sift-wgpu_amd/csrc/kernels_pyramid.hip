@@ -423,41 +423,50 @@ __global__ __launch_bounds__(kThreads) void k_resize_v(const ResizeJob* __restri
 }
 
 constexpr int kResizeSpan = 2 * kThreads + 64;  // input floats for 256 outputs (ratio ~2)
+constexpr int kResizeRows = 8;                   // output rows per workgroup (share weights)
 
 __global__ __launch_bounds__(kThreads) void k_resize_h(const ResizeJob* __restrict__ jobs,
                                                        int njobs) {
-    __shared__ float span[kResizeSpan];
+    __shared__ float span[kResizeRows][kResizeSpan];
     const int tile = blockIdx.x;
     const ResizeJob& J = jobs[find_job(jobs, njobs, tile)];
     const int t = tile - J.tile_begin;
-    const int y = t / J.tiles_x;
+    const int y0 = (t / J.tiles_x) * kResizeRows;
     const int ox0 = (t % J.tiles_x) * kThreads;
     const int ox = ox0 + threadIdx.x;
     const int oxl = min(ox0 + kThreads, J.out_w) - 1;
-    const float* src = J.tmp + (size_t)y * J.tmp_pitch;
+    const int nrows = min(kResizeRows, J.out_h - y0);
     const int lo = J.left[ox0];
     const int hi = J.left[oxl] + J.count[oxl];  // exclusive
     const bool staged = hi - lo <= kResizeSpan;
-    if (staged)
-        for (int i = threadIdx.x; i < hi - lo; i += kThreads) span[i] = src[lo + i];
+    if (staged) {
+        for (int r = 0; r < nrows; r++) {
+            const float* src = J.tmp + (size_t)(y0 + r) * J.tmp_pitch;
+            for (int i = threadIdx.x; i < hi - lo; i += kThreads) span[r][i] = src[lo + i];
+        }
+    }
     __syncthreads();
     if (ox >= J.out_w) return;
     const int l = J.left[ox], n = J.count[ox];
     const float* wr = J.weights + (size_t)ox * J.maxtaps;
-    const float* in = staged ? span + (l - lo) : src + l;
-    float s = 0.0f;
+    float wv[16];  // this column's weights, loaded once for all rows
     if (n <= 16) {
-        float wv[16];  // issue all weight loads before the dependent sum
 #pragma unroll
         for (int i = 0; i < 16; i++) wv[i] = i < n ? wr[i] : 0.0f;
-#pragma unroll
-        for (int i = 0; i < 16; i++)
-            if (i < n) s = mul_add_exact(s, in[i], wv[i]);
-    } else {
-        for (int i = 0; i < n; i++) s = mul_add_exact(s, in[i], wr[i]);
     }
-    float c = s < 0.0f ? 0.0f : (s > 255.0f ? 255.0f : s);
-    J.dst[(size_t)y * J.dst_pitch + ox] = (uint8_t)roundf(c);
+    for (int r = 0; r < nrows; r++) {
+        const float* in = staged ? &span[r][l - lo] : J.tmp + (size_t)(y0 + r) * J.tmp_pitch + l;
+        float s = 0.0f;
+        if (n <= 16) {
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                if (i < n) s = mul_add_exact(s, in[i], wv[i]);
+        } else {
+            for (int i = 0; i < n; i++) s = mul_add_exact(s, in[i], wr[i]);
+        }
+        float c = s < 0.0f ? 0.0f : (s > 255.0f ? 255.0f : s);
+        J.dst[(size_t)(y0 + r) * J.dst_pitch + ox] = (uint8_t)roundf(c);
+    }
 }
 
 }  // namespace

@@ -7,6 +7,8 @@
 // sift_hip_detect_batch, and post completions. Frames that arrive while a batch
 // is on the device go into the next batch of whichever lane frees first, so two
 // batches overlap on the device and queueing stays short under load.
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <chrono>
@@ -63,6 +65,7 @@ struct Request {
 };
 
 struct Server {
+    bool trace = false;  // SIFT_HIP_TRACE=1: one stderr line per batch
     Context* owner = nullptr;
     sift_server_config_t cfg{};
     std::mutex mu;
@@ -109,6 +112,12 @@ void Server::worker(Context* lane) {
         std::vector<sift_result_t> res(batch.size());
         int rc = run_and_collect(lane, B, res.data());
         const double end = now_ms();
+        if (trace) {
+            double oldest = start;
+            for (const Request& r : batch) oldest = std::min(oldest, r.submit_ms);
+            fprintf(stderr, "[sift-hip server] lane %p batch %zu frames %.1f MB queued %.3f ms run %.3f ms rc %d\n",
+                    (void*)lane, batch.size(), batch_bytes / 1e6, start - oldest, end - start, rc);
+        }
         {
             std::lock_guard<std::mutex> lk(mu);
             for (size_t i = 0; i < batch.size(); i++) {
@@ -142,6 +151,8 @@ int sift_hip_server_start(sift_hip_ctx* ctx, const sift_server_config_t* cfg) {
     if (!S) return SIFT_ERR_NOMEM;
     S->owner = C;
     S->cfg = *cfg;
+    const char* tr = getenv("SIFT_HIP_TRACE");
+    S->trace = tr && tr[0] == '1';
     if (S->cfg.max_batch == 0) S->cfg.max_batch = 64;
     if (S->cfg.max_queue == 0) S->cfg.max_queue = 4096;
     if (S->cfg.hbm_budget_bytes == 0) S->cfg.hbm_budget_bytes = (size_t)32 << 30;
@@ -152,6 +163,16 @@ int sift_hip_server_start(sift_hip_ctx* ctx, const sift_server_config_t* cfg) {
         L->params = C->params;
         if (hipSetDevice(C->device) != hipSuccess ||
             hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete L;
+            break;
+        }
+        // preallocated device pool of the lane: room for a full batch of large
+        // frames, so the arena does not grow (and synchronise) mid-stream
+        // and pinned staging for a batch of a few 4K frames each way
+        const size_t pool = std::min(S->cfg.hbm_budget_bytes, (size_t)6 << 30);
+        if (L->ensure_arena(pool) != SIFT_OK || L->ensure_pinned((size_t)64 << 20) != SIFT_OK ||
+            L->ensure_pinned_out((size_t)64 << 20) != SIFT_OK) {
+            L->release();
             delete L;
             break;
         }
@@ -236,13 +257,7 @@ int sift_hip_server_stop(sift_hip_ctx* ctx) {
     for (auto& t : S->workers) t.join();
     for (auto& c : S->done) sift_result_free(&c.result);
     for (Context* L : S->lanes) {
-        (void)hipSetDevice(L->device);
-        if (L->stream) (void)hipStreamSynchronize(L->stream);
-        if (L->arena) (void)hipFree(L->arena);
-        if (L->tables.dev) (void)hipFree(L->tables.dev);
-        if (L->pinned) (void)hipHostFree(L->pinned);
-        for (hipEvent_t e : L->prof_events) (void)hipEventDestroy(e);
-        if (L->stream) (void)hipStreamDestroy(L->stream);
+        L->release();
         delete L;
     }
     delete S;

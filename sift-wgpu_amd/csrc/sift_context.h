@@ -45,13 +45,18 @@ struct ParamKey {
     }
 };
 
-// Host mirror + device copy of all weight tables, append-only.
+// Host mirror + device copy of all weight tables, append-only. New tables are
+// uploaded asynchronously on the owning stream through a pinned bounce buffer
+// (reused only after the stream has been synchronised, which every public call
+// does before returning).
 struct TableStore {
     std::vector<char> host;
     char* dev = nullptr;
     size_t dev_cap = 0, uploaded = 0;
+    char* bounce = nullptr;
+    size_t bounce_cap = 0;
     size_t append(const void* p, size_t bytes);
-    int ensure_uploaded();
+    int ensure_uploaded(hipStream_t stream);
 };
 
 struct Batch {
@@ -105,6 +110,8 @@ struct Context {
     size_t arena_cap = 0;
     char* pinned = nullptr;
     size_t pinned_cap = 0;
+    char* pinned_out = nullptr;  // device-to-host staging for results
+    size_t pinned_out_cap = 0;
     size_t cand_cap = 0, kp_cap = 0;
 
     Server* server = nullptr;  // owned; created by sift_hip_server_start
@@ -122,6 +129,8 @@ struct Context {
     const LanczosRef& lanczos_for(int in, int out);
     int ensure_arena(size_t bytes);
     int ensure_pinned(size_t bytes);
+    int ensure_pinned_out(size_t bytes);
+    void release();  // frees every device and pinned allocation of the context
     // Enqueues the whole pipeline for a batch on `stream` (no host sync except
     // for table uploads); outputs stay in the arena until the next run.
     int run(const Batch& B, RunOutputs* outs);
