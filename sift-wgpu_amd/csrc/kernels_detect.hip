@@ -5,6 +5,7 @@
 //   k_refine      refine_and_filter_extrema     src/sift.rs:299-474 (+ solve 478-503)
 //   k_orient      assign_orientations           src/sift.rs:605-726
 //   k_descriptor  compute + normalize           src/sift.rs:917-1060, 889-913
+//   k_descriptor_wide  the same for windows wider than the LDS patch
 //
 // DoG values are never stored: every DoG sample is recomputed from the u8
 // Gaussian levels as q[G_{s+1}] - q[G_s] with q[v] = v / 255 (IEEE division,
@@ -19,7 +20,7 @@ namespace {
 constexpr float kPi = 3.14159274101257324219f;  // std::f32::consts::PI
 constexpr float kTwoPi = 2.0f * kPi;
 constexpr float k8OverTwoPi = 8.0f / kTwoPi;
-constexpr float kDescFix = 2097152.0f;  // 2^21: u32 fixed point of the descriptor sums
+constexpr float kDescFix = 1048576.0f;  // 2^20: u32 fixed point of the descriptor sums
 constexpr int kThreads = 256;
 constexpr int kExtChunk = 1024;  // pixels of one row per chunk in k_extrema
 constexpr int kScanTile = 4096;  // 256 threads x 16 elements
@@ -678,251 +679,426 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-constexpr int kDescMaxR = 31;                         // sigma_octave <= 3.65
-constexpr int kDescGrid = (2 * kDescMaxR + 3) * (2 * kDescMaxR + 3);  // bilinear grid
+constexpr int kDescMaxR = 31;                   // LDS path for sigma_octave <= 3.65
+constexpr int kGS = 2 * kDescMaxR + 3;          // bilinear grid row stride (floats)
+constexpr int kPatchRows = 2 * kDescMaxR + 6;   // staged patch rows
+constexpr int kPS = 72;                         // staged patch row stride (bytes, >= rows + 3)
+constexpr int kCellWords = 10;                  // one histogram cell: 8 angle bins + pair padding
+constexpr int kTableWords = 36 * kCellWords;    // 6 x 6 padded cells
+constexpr int kHistWords = 2 * kTableWords;     // one copy: even-pair table + odd-pair table
+constexpr int kHistCopies = 2;                  // waves {0, 1} and {2, 3} share a copy
+constexpr int kMaxCopySamples = 2048;           // per-copy sample bound that keeps u32 sums exact
 
-// Trilinear scatter of one sample (src/sift.rs:1012-1049) into a u32 fixed-point
-// histogram padded to 6x6 spatial cells (cells -1..4 in x and y) x 8 angles:
-// the reference drops neighbours outside the 4x4 grid; here they land in the
-// border cells, which are never read, so no clamping or branching is needed and
-// the four spatial neighbours are constant offsets (LDS instruction offsets).
-// Contribution ((wm * wx) * wy) * wa in the reference's product order; wm_fix =
-// weighted magnitude * 2^21; xb, yb in (-1, 4), ab in [0, 8]. Integer adds make
-// the sums independent of the order.
-constexpr int kDescPadBins = 6 * 6 * 8;
-
-__device__ __forceinline__ int desc_bin(int iy, int ix, int ia) {  // ix, iy in 0..3
-    return ((iy + 1) * 6 + (ix + 1)) * 8 + ia;
+// Trilinear scatter of one sample (src/sift.rs:1012-1049), contribution
+// ((m * wx) * wy) * wa in u32 fixed point, m = weighted magnitude * fix.
+// The histogram is padded to 6 x 6 spatial cells (cells -1..4 in x and y): the
+// reference drops neighbours outside the 4 x 4 grid, here they land in border
+// cells that are never read, so there is no clamping and the four spatial
+// neighbours are constant offsets. The two angle bins (a, a+1 mod 8) of each
+// spatial neighbour go in ONE 64-bit LDS add: the even-pair table holds bins
+// 0..7 in slots 0..7 (pairs (0,1), (2,3), ... are 8-byte aligned); the
+// odd-pair table holds bin b in slot b+1 and bin 0 again in slot 9, so pairs
+// (1,2), (3,4), (5,6), (7,0) are aligned too. Every bin sum of a copy stays
+// below 2^32 (at most kMaxCopySamples samples of at most sqrt(2) * fix + 1/2
+// each), so the low word never carries into the high word.
+__device__ __forceinline__ void desc_scatter(unsigned* hcopy, float xb, float yb, float ab,
+                                             float m) {
+    const float xf = floorf(xb), yf = floorf(yb), af = floorf(ab);
+    const int ai = ((int)af) & 7, p = ai & 1;
+    const float dx = xb - xf, dy = yb - yf, da = ab - af;
+    // word offset of cell (xf + 1, yf + 1), xf, yf in -1..3: small exact integers
+    const int cw = (int)__builtin_fmaf(yf, 6.0f * kCellWords,
+                                       __builtin_fmaf(xf, (float)kCellWords, 7.0f * kCellWords));
+    unsigned long long* base =
+        reinterpret_cast<unsigned long long*>(hcopy + p * (kTableWords + 1) + cw + ai);
+    const float t0 = m * (1.0f - dx), t1 = m * dx;
+    const float tt[4] = {t0 * (1.0f - dy), t1 * (1.0f - dy), t0 * dy, t1 * dy};
+    constexpr int off[4] = {0, kCellWords / 2, 6 * kCellWords / 2, 7 * kCellWords / 2};
+    const float wa0 = 1.0f - da;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const unsigned lo = (unsigned)__builtin_fmaf(tt[k], wa0, 0.5f);
+        const unsigned hi = (unsigned)__builtin_fmaf(tt[k], da, 0.5f);
+        atomicAdd(base + off[k], ((unsigned long long)hi << 32) | lo);
+    }
 }
 
-__device__ __forceinline__ void desc_scatter(unsigned* hist, float xb, float yb, float ab,
-                                             float wm_fix) {
-    const float xf = floorf(xb), yf = floorf(yb), af = floorf(ab);
-    const int xbi = (int)xf, ybi = (int)yf, abi = (int)af;
-    const float dxi = xb - xf, dyi = yb - yf, dai = ab - af;
-    const float wx[2] = {1.0f - dxi, dxi};
-    const float wy[2] = {1.0f - dyi, dyi};
-    const float wa[2] = {1.0f - dai, dai};
-    unsigned* cell = hist + ((ybi + 1) * 6 + (xbi + 1)) * 8;
-    const int ia[2] = {abi & 7, (abi + 1) & 7};
+// Interval of px where c * px + d lies in (-B, B) in real arithmetic; the
+// caller widens it by one sample and the per-sample window test decides.
+// A coefficient too small to move c * px by 0.1% of B over the window keeps the
+// whole row or none of it.
+__device__ __forceinline__ void desc_row_interval(float c, float d, float B, int R, float& lo,
+                                                  float& hi) {
+    if (fabsf(c) * (float)(R + 1) <= 1e-3f * B) {
+        const bool keep = fabsf(d) < B * 1.002f;
+        lo = keep ? -1e6f : 1e6f;
+        hi = keep ? 1e6f : -1e6f;
+        return;
+    }
+    const float e1 = (-B - d) / c, e2 = (B - d) / c;
+    lo = fminf(e1, e2);
+    hi = fmaxf(e1, e2);
+}
+
+// Per-keypoint geometry of compute (src/sift.rs:940-960).
+struct DescGeom {
+    Keypoint base;
+    const uint8_t* g;
+    int w, h, pitch, R, side, np;
+    unsigned k0;
+    float xo, yo, so, bw, denom;
+};
+
+__device__ __forceinline__ DescGeom desc_geom(int c, const OctaveDesc* __restrict__ oct,
+                                              const ImageParams* __restrict__ params,
+                                              const Cand* __restrict__ cand,
+                                              const Keypoint* __restrict__ ref,
+                                              const unsigned* __restrict__ npeaks,
+                                              const unsigned* __restrict__ koff) {
+    DescGeom G;
+    G.np = (int)npeaks[c];
+    G.base = ref[c];
+    G.k0 = koff[c];
+    const OctaveDesc& O = oct[cand[c].job];
+    const int S = params[O.img].num_intervals;
+    const int gl = min(max(G.base.layer, 0), S + 2);
+    G.g = O.levels + (long long)gl * O.level_stride;
+    G.w = O.w;
+    G.h = O.h;
+    G.pitch = O.pitch;
+    const float scale = pow2f(G.base.octave);
+    G.xo = G.base.x / scale;
+    G.yo = G.base.y / scale;
+    G.so = G.base.size / scale;
+    G.bw = 3.0f * G.so;
+    const float win = G.bw * 4.0f;
+    const float wsig = 0.5f * win;
+    G.denom = 2.0f * wsig * wsig;
+    G.R = G.so > 0.0f ? f2i_sat(ceilf(win * sqrtf(2.0f) * 0.5f)) : 0;
+    G.side = 2 * G.R + 1;
+    return G;
+}
+
+// (ix, iy) range whose x_o + (ix - R) lies in [0, w-1) and y_o + (iy - R) in
+// [0, h-1) (src/sift.rs:981-987), evaluated with the reference's f32 test at
+// the candidate integers.
+__device__ __forceinline__ void desc_bounds(const DescGeom& G, int& bx0, int& bx1, int& by0,
+                                            int& by1) {
+    const float wm1 = (float)(G.w - 1), hm1 = (float)(G.h - 1);
+    bx0 = 0;
+    bx1 = G.side - 1;
+    by0 = 0;
+    by1 = G.side - 1;
+    while (bx0 <= bx1 && G.xo + (float)(bx0 - G.R) < 0.0f) bx0++;
+    while (bx1 >= bx0 && G.xo + (float)(bx1 - G.R) >= wm1) bx1--;
+    while (by0 <= by1 && G.yo + (float)(by0 - G.R) < 0.0f) by0++;
+    while (by1 >= by0 && G.yo + (float)(by1 - G.R) >= hm1) by1--;
+}
+
+// Wave 0: sums the histogram copies of one peak, normalises, clips at 0.2,
+// renormalises (src/sift.rs:889-913) and writes the descriptor and keypoint.
+__device__ __forceinline__ void desc_finish(const unsigned* hist, float fix, int l,
+                                            const Keypoint& base, float angle, unsigned kidx,
+                                            int kcap, Keypoint* __restrict__ kps_out,
+                                            float* __restrict__ desc_out) {
+    float v[2];
 #pragma unroll
-    for (int i = 0; i < 2; i++) {
-        const float t = wm_fix * wx[i];
+    for (int e = 0; e < 2; e++) {
+        const int dd = 2 * l + e;  // (iy * 4 + ix) * 8 + ia
+        const int a = dd & 7;
+        const int cell = ((dd >> 5) + 1) * 6 + (((dd >> 3) & 3) + 1);
+        unsigned long long sum = 0;
 #pragma unroll
-        for (int jj = 0; jj < 2; jj++) {
-            const float tt = t * wy[jj];
-#pragma unroll
-            for (int k = 0; k < 2; k++)
-                atomicAdd(&cell[(jj * 6 + i) * 8 + ia[k]], (unsigned)(tt * wa[k] + 0.5f));
+        for (int cp = 0; cp < kHistCopies; cp++) {
+            const unsigned* t = hist + cp * kHistWords + cell * kCellWords;
+            sum += t[a];                    // even-pair table, slot a
+            sum += t[kTableWords + a + 1];  // odd-pair table, slot a + 1
+            if (a == 0) sum += t[kTableWords + 9];
+        }
+        v[e] = (float)((double)sum / (double)fix);
+    }
+    float v0 = v[0], v1 = v[1];
+    const float nrm = sqrtf(wave_sum(v0 * v0 + v1 * v1));
+    if (nrm >= 1e-8f) {
+        const float inv = 1.0f / nrm;
+        v0 = fminf(v0 * inv, 0.2f);
+        v1 = fminf(v1 * inv, 0.2f);
+        const float n2 = sqrtf(wave_sum(v0 * v0 + v1 * v1));
+        if (n2 >= 1e-8f) {
+            const float inv2 = 1.0f / n2;
+            v0 *= inv2;
+            v1 *= inv2;
+        }
+    }
+    if (kidx < (unsigned)kcap) {
+        float2* dd = (float2*)(desc_out + (size_t)kidx * 128);
+        dd[l] = make_float2(v0, v1);
+        if (l == 0) {
+            Keypoint o = base;
+            o.angle = angle;
+            kps_out[kidx] = o;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// k_descriptor: one workgroup (256) per refined keypoint, looping over its
-// orientation peaks (src/sift.rs:917-1060).
-// Per keypoint: the clamped u8 patch of the Gaussian level around (x_o, y_o) is
-// staged in LDS (every texel the reference's get_gauss_pixel_bilinear can touch
-// in the window, with its clamp-to-edge rule), and the bilinear samples on the
-// shifted grid (x_o + i, y_o + j), |i|,|j| <= R+1, are formed once from it. A
-// window sample (px, py) takes its gradient from the four grid neighbours
-// (the reference's bilinear(x_s +- 1, y_s) and bilinear(x_s, y_s +- 1)).
-// Per peak, each sample is rotated and binned; in-window samples compute
-// mag * exp(-(px^2+py^2)/(2 (6 sigma_o)^2)) and atan2 and scatter trilinearly
-// into a per-wave u32 fixed-point histogram (integer adds: independent of the
-// order; per-wave sums stay below 2^32 for any window of at most 4096 samples).
-// The four wave histograms are summed and wave 0 normalises, clips at
-// 0.2 and renormalises (src/sift.rs:889-913).
-// Keypoints whose window exceeds the LDS patch (sigma_octave > 3.65) use the
-// direct per-sample path with global reads.
+// k_descriptor: one workgroup (256) per refined keypoint with R <= 31
+// (sigma_octave <= 3.65), looping over its orientation peaks
+// (src/sift.rs:917-1060).
+// Per keypoint: the clamped u8 patch of the Gaussian level around (x_o, y_o) --
+// every texel the reference's get_gauss_pixel_bilinear can touch in the
+// window, with its clamp-to-edge rule -- is staged in LDS (aligned 32-bit
+// copies when the patch is inside the level, clamped byte loads at the
+// borders), and the bilinear samples on the shifted grid (x_o + i, y_o + j),
+// |i|,|j| <= R+1, are formed once from it. A window sample (px, py) takes its
+// gradient from the four grid neighbours (the reference's bilinear(x_s +- 1, y_s)
+// and bilinear(x_s, y_s +- 1)).
+// Per peak, the samples of each row whose rotated bin coordinates can fall in
+// (-1, 4) form one interval (the rotated window is convex); the intervals,
+// widened by one sample and clipped to the in-image rectangle, are laid end to
+// end and split into 256 contiguous chunks, one per thread, so nearly every lane
+// works on an in-window sample and the lanes of a wave sit far apart in the
+// window (different histogram cells). Each sample still applies the window
+// test xb, yb in (-1, 4), then computes mag * exp(-(px^2+py^2)/(2 (6 sigma_o)^2))
+// and atan2 and scatters trilinearly into its copy of the histogram (integer
+// adds: independent of the order). Wave 0 sums the copies and normalises.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_descriptor(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_descriptor(
     const OctaveDesc* __restrict__ oct, const ImageParams* __restrict__ params,
     const Cand* __restrict__ cand, const unsigned* __restrict__ total, int cap,
     const Keypoint* __restrict__ ref, const unsigned* __restrict__ npeaks,
     const float* __restrict__ angles, const unsigned* __restrict__ koff, int kcap,
     Keypoint* __restrict__ kps_out, float* __restrict__ desc_out) {
-    constexpr int kPatch = 2 * kDescMaxR + 6;
-    // The u8 patch is dead once the bilinear grid is built; the histograms are
-    // live only after that, so they share storage.
-    constexpr int kUnion = (kPatch * kPatch + 3) / 4 > 4 * kDescPadBins ? (kPatch * kPatch + 3) / 4
-                                                                         : 4 * kDescPadBins;
+    static_assert(kHistCopies * kHistWords * 4 >= kPatchRows * kPS, "patch shares the histograms");
     __shared__ float q[256];
-    __shared__ unsigned scratch_u[kUnion];
-    __shared__ float grid[kDescGrid];
-    __shared__ float ex[kDescMaxR + 2];
-    uint8_t* const patch = reinterpret_cast<uint8_t*>(scratch_u);
-    unsigned (*const hist)[kDescPadBins] = reinterpret_cast<unsigned (*)[kDescPadBins]>(scratch_u);
+    // the u8 patch is dead once the grid is built; the histograms live after that
+    __shared__ __align__(16) unsigned pu[kHistCopies * kHistWords];
+    __shared__ float grid[kGS * kGS];
+    __shared__ float exs[2 * kDescMaxR + 1];
+    __shared__ float4 rrec[64];  // per row: first and last column (int bits), sin * py, cos * py
+    __shared__ float rwy[64];
+    __shared__ int rpre[65];
+    uint8_t* const patch = reinterpret_cast<uint8_t*>(pu);
+    unsigned* const hist = pu;
     fill_q(q);
     __syncthreads();
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int n = (int)min(*total, (unsigned)cap);
     for (int c = blockIdx.x; c < n; c += gridDim.x) {
-        const int np = (int)npeaks[c];
-        if (np == 0) continue;
-        const Keypoint base = ref[c];
-        const OctaveDesc& O = oct[cand[c].job];
-        const ImageParams& P = params[O.img];
-        const int S = P.num_intervals;
-        const int gl = min(max(base.layer, 0), S + 2);
-        const uint8_t* g = O.levels + (long long)gl * O.level_stride;
-        const int w = O.w, h = O.h, pitch = O.pitch;
-        const float scale = pow2f(base.octave);
-        const float xo = base.x / scale, yo = base.y / scale, so = base.size / scale;
-        const unsigned k0 = koff[c];
-        const float bw = 3.0f * so;
-        const float win = bw * 4.0f;
-        const float wsig = 0.5f * win;
-        const float denom = 2.0f * wsig * wsig;
-        const int R = so > 0.0f ? f2i_sat(ceilf(win * sqrtf(2.0f) * 0.5f)) : 0;
-        const bool lds_path = so > 0.0f && R <= kDescMaxR;
-        const int side = 2 * R + 1, gside = 2 * R + 3;
-        const int nsamp = side * side;
-        const float wm1 = (float)(w - 1), hm1 = (float)(h - 1);
-        if (lds_path) {
-            // patch texel (u, v) = G(clamp(px0 + u), clamp(py0 + v)); every bilinear
-            // tap of the grid, floor(x_o +- (R+1)) .. +1, lies in [px0, px0 + pside)
+        const DescGeom G = desc_geom(c, oct, params, cand, ref, npeaks, koff);
+        if (G.np == 0 || G.R > kDescMaxR) continue;  // wider windows: k_descriptor_wide
+        const int R = G.R, side = G.side, gside = 2 * R + 3;
+        const float xo = G.xo, yo = G.yo, bw = G.bw, denom = G.denom;
+        {
             const int pside = 2 * R + 6;
             const int px0 = f2i_sat(floorf(xo)) - R - 2, py0 = f2i_sat(floorf(yo)) - R - 2;
-            for (int i = tid; i < pside * pside; i += kThreads) {
-                const int v = i / pside, u = i - v * pside;
-                const int xx = min(max(px0 + u, 0), w - 1), yy = min(max(py0 + v, 0), h - 1);
-                patch[i] = g[(size_t)yy * pitch + xx];
+            const int px0a = px0 & ~3, poff = px0 - px0a;
+            if (px0 >= 0 && py0 >= 0 && px0 + pside <= G.w && py0 + pside <= G.h) {
+                // interior: aligned words [px0a, round_up(px0 + pside, 4)) of each row,
+                // which end within the row's pitch
+                const int nw = (poff + pside + 3) >> 2;  // <= kPS / 4
+                const int tot = nw * pside;              // <= 18 * 68 < 5 * 256
+                const float inv_nw = 1.0f / (float)nw;
+                const uint8_t* rowp = G.g + (size_t)py0 * G.pitch + px0a;
+                unsigned wv[5];
+#pragma unroll
+                for (int k = 0; k < 5; k++) {
+                    const int i = tid + k * kThreads;
+                    if (i < tot) {
+                        const int v = (int)(((float)i + 0.5f) * inv_nw);
+                        const int wd = i - v * nw;
+                        wv[k] = *reinterpret_cast<const unsigned*>(rowp + (size_t)v * G.pitch + 4 * wd);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 5; k++) {
+                    const int i = tid + k * kThreads;
+                    if (i < tot) {
+                        const int v = (int)(((float)i + 0.5f) * inv_nw);
+                        pu[v * (kPS / 4) + (i - v * nw)] = wv[k];
+                    }
+                }
+            } else {
+                const float inv_ps = 1.0f / (float)pside;
+                for (int i = tid; i < pside * pside; i += kThreads) {
+                    const int v = (int)(((float)i + 0.5f) * inv_ps), u = i - v * pside;
+                    const int xx = min(max(px0 + u, 0), G.w - 1), yy = min(max(py0 + v, 0), G.h - 1);
+                    patch[v * kPS + poff + u] = G.g[(size_t)yy * G.pitch + xx];
+                }
             }
             __syncthreads();
+            const float inv_gs = 1.0f / (float)gside;
             for (int i = tid; i < gside * gside; i += kThreads) {
-                const int gy = i / gside, gx = i - gy * gside;
+                const int gy = (int)(((float)i + 0.5f) * inv_gs), gx = i - gy * gside;
                 const float x = xo + (float)(gx - R - 1), y = yo + (float)(gy - R - 1);
                 const float xf = floorf(x), yf = floorf(y);
                 const float dx = x - xf, dy = y - yf;
                 const int u0 = min(max(f2i_sat(xf) - px0, 0), pside - 2);
                 const int v0 = min(max(f2i_sat(yf) - py0, 0), pside - 2);
-                const uint8_t* pp = patch + v0 * pside + u0;
+                const uint8_t* pp = patch + v0 * kPS + poff + u0;
                 const float q11 = q[pp[0]], q21 = q[pp[1]];
-                const float q12 = q[pp[pside]], q22 = q[pp[pside + 1]];
-                grid[i] = q11 * (1.0f - dx) * (1.0f - dy) + q21 * dx * (1.0f - dy) +
-                          q12 * (1.0f - dx) * dy + q22 * dx * dy;
+                const float q12 = q[pp[kPS]], q22 = q[pp[kPS + 1]];
+                grid[gy * kGS + gx] = q11 * (1.0f - dx) * (1.0f - dy) + q21 * dx * (1.0f - dy) +
+                                      q12 * (1.0f - dx) * dy + q22 * dx * dy;
             }
+            // Gaussian weight exp(-(px^2+py^2)/denom) as exp(-px^2/denom) * exp(-py^2/denom)
+            for (int i = tid; i < side; i += kThreads) exs[i] = expf(-(float)((i - R) * (i - R)) / denom);
         }
         __syncthreads();  // patch reads done before the histograms reuse its storage
-        const int per = (nsamp + kThreads - 1) / kThreads;
-        const int chunk0 = (lane * 4 + wave) * per;  // lanes of a wave spread over the window
-        // Gaussian weight exp(-(px^2+py^2)/denom) as exp(-px^2/denom) * exp(-py^2/denom)
-        if (lds_path)
-            for (int i = tid; i <= R; i += kThreads) ex[i] = expf(-(float)(i * i) / denom);
-        // (ix, iy) range whose x_o + (ix - R) lies in [0, w-1) and y_o + (iy - R) in [0, h-1),
-        // evaluated with the reference's f32 test at the candidate integers
-        int bx0 = 0, bx1 = side - 1, by0 = 0, by1 = side - 1;
-        while (bx0 <= bx1 && xo + (float)(bx0 - R) < 0.0f) bx0++;
-        while (bx1 >= bx0 && xo + (float)(bx1 - R) >= wm1) bx1--;
-        while (by0 <= by1 && yo + (float)(by0 - R) < 0.0f) by0++;
-        while (by1 >= by0 && yo + (float)(by1 - R) >= hm1) by1--;
-        for (int pk = 0; pk < np; pk++) {
+        int bx0, bx1, by0, by1;
+        desc_bounds(G, bx0, bx1, by0, by1);
+        const int nrows = G.so > 0.0f ? max(by1 - by0 + 1, 0) : 0;  // <= 63
+        for (int pk = 0; pk < G.np; pk++) {
             const float angle = angles[(size_t)c * kMaxPeaks + pk];
-            for (int i = tid; i < 4 * kDescPadBins; i += kThreads) (&hist[0][0])[i] = 0u;
-            __syncthreads();
-            if (so > 0.0f) {
-                const float ca = cosf(angle), sa = sinf(angle);
-                const float inv_bw = 1.0f / bw;
-                unsigned* hw = hist[wave];
-                if (lds_path) {
-                    // Lane l owns block (l / 8, l % 8) of an 8 x 8 split of the window;
-                    // wave w takes rows w, w+4, ... of each block, so at every step the
-                    // 64 lanes sit on a lattice spread over the whole window (different
-                    // histogram cells). Samples with x_o + px in [0, w-1) and y_o + py in
-                    // [0, h-1) (src/sift.rs:981-987) form the rectangle [bx0,bx1] x
-                    // [by0,by1], which bounds the loops. -sin * px is computed as
-                    // (-sin) * px, the same value.
-                    const int rs = (side + 7) >> 3;
-                    const int oy = (lane >> 3) * rs, oxx = (lane & 7) * rs;
-                    const int kx0 = max(0, bx0 - oxx), kx1 = min(rs - 1, bx1 - oxx);
-                    const float msa = -sa;
-                    for (int ky = wave; ky < rs; ky += 4) {
-                        const int cy = oy + ky;
-                        if (cy < by0 || cy > by1) continue;
-                        const float py = (float)(cy - R);
-                        const float sy = sa * py, cy_ = ca * py;  // per-row terms of rx, ry
-                        const float wy = ex[abs(cy - R)];
-                        const float* gr = grid + (cy + 1) * gside + (oxx + kx0 + 1);
-#pragma unroll 2
-                        for (int kx = kx0; kx <= kx1; kx++, gr++) {
-                            const int cx = oxx + kx;
-                            const float px = (float)(cx - R);
-                            const float rx = ca * px + sy;
-                            const float ry = msa * px + cy_;
-                            const float xb = rx * inv_bw + 4.0f / 2.0f - 0.5f;
-                            const float yb = ry * inv_bw + 4.0f / 2.0f - 0.5f;
-                            if (!(xb > -1.0f && xb < 4.0f && yb > -1.0f && yb < 4.0f)) continue;
-                            const float gx = gr[1] - gr[-1];
-                            const float gyv = gr[gside] - gr[-gside];
-                            const float wt = ex[abs(cx - R)] * wy;
-                            const float mag = __builtin_amdgcn_sqrtf(gx * gx + gyv * gyv);
-                            const float pa = atan2_grad(gyv, gx);
-                            const float d = pa - angle;  // in [-2pi, 2pi): rem_euclid(d, 2pi)
-                            const float ar = d < 0.0f ? d + kTwoPi : d;
-                            desc_scatter(hw, xb, yb, ar * k8OverTwoPi, mag * wt * kDescFix);
-                        }
-                    }
-                } else {
-                    const int iend = min(chunk0 + per, nsamp);
-                    for (int i = chunk0; i < iend; i++) {
-                        const int cy = i / side, cx = i - cy * side;
-                        const float py = (float)(cy - R), px = (float)(cx - R);
-                        const float rx = ca * px + sa * py;
-                        const float ry = -sa * px + ca * py;
-                        const float xb = rx / bw + 4.0f / 2.0f - 0.5f;
-                        const float yb = ry / bw + 4.0f / 2.0f - 0.5f;
-                        if (!(xb > -1.0f && xb < 4.0f && yb > -1.0f && yb < 4.0f)) continue;
-                        const float xs = xo + px, ys = yo + py;
-                        if (xs < 0.0f || xs >= wm1 || ys < 0.0f || ys >= hm1) continue;
-                        const float gx = bilinear(g, pitch, w, h, xs + 1.0f, ys, q) -
-                                         bilinear(g, pitch, w, h, xs - 1.0f, ys, q);
-                        const float gy = bilinear(g, pitch, w, h, xs, ys + 1.0f, q) -
-                                         bilinear(g, pitch, w, h, xs, ys - 1.0f, q);
-                        const float wt = expf(-(px * px + py * py) / denom);
-                        const float mag = sqrtf(gx * gx + gy * gy);
-                        const float pa = atan2f(gy, gx);
-                        const float ar = rem_euclid(pa - angle, kTwoPi);
-                        desc_scatter(hw, xb, yb, ar * 8.0f / kTwoPi, mag * wt * kDescFix);
-                    }
-                }
-            }
-            __syncthreads();
+            for (int i = tid; i < kHistCopies * kHistWords / 4; i += kThreads)
+                reinterpret_cast<uint4*>(hist)[i] = make_uint4(0u, 0u, 0u, 0u);
+            const float ca = cosf(angle), sa = sinf(angle);
+            const float inv_bw = 1.0f / bw;
+            const float msa = -sa;  // -sin * px is computed as (-sin) * px, the same value
             if (tid < 64) {
-                const int l = tid;
-                float v[2];
+                // per-row sample intervals of the rotated window, prefix-summed
+                int lo = 0, cnt = 0;
+                if (tid < nrows) {
+                    const int cy = by0 + tid;
+                    const float py = (float)(cy - R);
+                    const float B = 2.5f * bw;
+                    float l1, h1, l2, h2;
+                    desc_row_interval(ca, sa * py, B, R, l1, h1);
+                    desc_row_interval(msa, ca * py, B, R, l2, h2);
+                    const float lof = fmaxf(fmaxf(l1, l2), -1e5f), hif = fminf(fminf(h1, h2), 1e5f);
+                    const int a = max((int)ceilf(lof) + R - 1, bx0);
+                    const int b = min((int)floorf(hif) + R + 1, bx1);
+                    lo = a;
+                    cnt = max(b - a + 1, 0);
+                    rrec[tid] = make_float4(__int_as_float(lo), __int_as_float(lo + cnt - 1), sa * py, ca * py);
+                    rwy[tid] = expf(-(float)((cy - R) * (cy - R)) / denom) * kDescFix;
+                }
+                int incl = cnt;
 #pragma unroll
-                for (int e = 0; e < 2; e++) {
-                    const int d = 2 * l + e;  // (iy * 4 + ix) * 8 + ia
-                    const int b = desc_bin(d >> 5, (d >> 3) & 3, d & 7);
-                    const unsigned long long sum = (unsigned long long)hist[0][b] + hist[1][b] +
-                                                   hist[2][b] + hist[3][b];
-                    v[e] = (float)((double)sum * (1.0 / (double)kDescFix));
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int t = __shfl_up(incl, d, 64);
+                    if (lane >= d) incl += t;
                 }
-                float v0 = v[0], v1 = v[1];
-                const float nrm = sqrtf(wave_sum(v0 * v0 + v1 * v1));
-                if (nrm >= 1e-8f) {
-                    const float inv = 1.0f / nrm;
-                    v0 = fminf(v0 * inv, 0.2f);
-                    v1 = fminf(v1 * inv, 0.2f);
-                    const float n2 = sqrtf(wave_sum(v0 * v0 + v1 * v1));
-                    if (n2 >= 1e-8f) {
-                        const float inv2 = 1.0f / n2;
-                        v0 *= inv2;
-                        v1 *= inv2;
-                    }
+                rpre[tid + 1] = incl;
+                if (tid == 0) rpre[0] = 0;
+            }
+            __syncthreads();
+            const int T = rpre[nrows];
+            const int per = (T + kThreads - 1) / kThreads;  // <= 16
+            int k = (lane * 4 + wave) * per;
+            const int kend = min(k + per, T);
+            if (k < kend) {
+                unsigned* hw = hist + (wave >> 1) * kHistWords;
+                int r0 = 0, r1 = nrows;  // rpre[r0] <= k < rpre[r1]
+                while (r1 - r0 > 1) {
+                    const int mid = (r0 + r1) >> 1;
+                    if (rpre[mid] <= k) r0 = mid;
+                    else r1 = mid;
                 }
-                const unsigned kidx = k0 + (unsigned)pk;
-                if (kidx < (unsigned)kcap) {
-                    float2* dd = (float2*)(desc_out + (size_t)kidx * 128);
-                    dd[l] = make_float2(v0, v1);
-                    if (l == 0) {
-                        Keypoint o = base;
-                        o.angle = angle;
-                        kps_out[kidx] = o;
+                int r = r0;
+                float4 rr = rrec[r];
+                int cx = __float_as_int(rr.x) + (k - rpre[r]);
+                int cxe = __float_as_int(rr.y);
+                float wyf = rwy[r];
+                const float* grow = grid + (by0 + r + 1) * kGS + 1;
+                for (; k < kend; k++, cx++) {
+                    if (cx > cxe) {  // next non-empty row (one exists: k < T)
+                        do {
+                            r++;
+                        } while (rpre[r + 1] == rpre[r]);
+                        rr = rrec[r];
+                        cx = __float_as_int(rr.x);
+                        cxe = __float_as_int(rr.y);
+                        wyf = rwy[r];
+                        grow = grid + (by0 + r + 1) * kGS + 1;
                     }
+                    // bin coordinates rx / bw + 4/2 - 0.5 (src/sift.rs:967-977), fused
+                    const float px = (float)(cx - R);
+                    const float rx = __builtin_fmaf(ca, px, rr.z);
+                    const float ry = __builtin_fmaf(msa, px, rr.w);
+                    const float xb = __builtin_fmaf(rx, inv_bw, 1.5f);
+                    const float yb = __builtin_fmaf(ry, inv_bw, 1.5f);
+                    if (!(xb > -1.0f && xb < 4.0f && yb > -1.0f && yb < 4.0f)) continue;
+                    // grid neighbours at non-negative offsets from (cx-1, cy-1)
+                    const float* gq = grow + cx - kGS - 1;
+                    const float gx = gq[kGS + 2] - gq[kGS];
+                    const float gyv = gq[2 * kGS + 1] - gq[1];
+                    const float mag = __builtin_amdgcn_sqrtf(__builtin_fmaf(gx, gx, gyv * gyv));
+                    const float m = mag * (exs[cx] * wyf);
+                    const float pa = atan2_grad(gyv, gx);
+                    const float d = pa - angle;  // in [-2pi, 2pi): rem_euclid(d, 2pi)
+                    const float ar = d < 0.0f ? d + kTwoPi : d;
+                    desc_scatter(hw, xb, yb, ar * k8OverTwoPi, m);
                 }
             }
+            __syncthreads();
+            if (tid < 64)
+                desc_finish(hist, kDescFix, tid, G.base, angle, G.k0 + (unsigned)pk, kcap, kps_out,
+                            desc_out);
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_descriptor_wide: keypoints whose window exceeds the LDS patch
+// (R > 31, sigma_octave > 3.65; none with the default sigma and 3 intervals).
+// Direct per-sample path with global reads and the reference's f32 arithmetic;
+// 256 contiguous chunks of the (2R+1)^2 window as in k_descriptor. The fixed-point
+// scale drops below 2^20 when a copy would see more than kMaxCopySamples samples.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_descriptor_wide(
+    const OctaveDesc* __restrict__ oct, const ImageParams* __restrict__ params,
+    const Cand* __restrict__ cand, const unsigned* __restrict__ total, int cap,
+    const Keypoint* __restrict__ ref, const unsigned* __restrict__ npeaks,
+    const float* __restrict__ angles, const unsigned* __restrict__ koff, int kcap,
+    Keypoint* __restrict__ kps_out, float* __restrict__ desc_out) {
+    __shared__ float q[256];
+    __shared__ __align__(16) unsigned hist[kHistCopies * kHistWords];
+    fill_q(q);
+    __syncthreads();
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int n = (int)min(*total, (unsigned)cap);
+    for (int c = blockIdx.x; c < n; c += gridDim.x) {
+        const DescGeom G = desc_geom(c, oct, params, cand, ref, npeaks, koff);
+        if (G.np == 0 || G.R <= kDescMaxR) continue;  // k_descriptor covers these
+        const int R = G.R, side = G.side, nsamp = side * side;
+        const float wm1 = (float)(G.w - 1), hm1 = (float)(G.h - 1);
+        const int per = (nsamp + kThreads - 1) / kThreads;
+        const float fix = kDescFix * fminf(1.0f, (float)kMaxCopySamples / (float)(128 * per));
+        for (int pk = 0; pk < G.np; pk++) {
+            const float angle = angles[(size_t)c * kMaxPeaks + pk];
+            for (int i = tid; i < kHistCopies * kHistWords / 4; i += kThreads)
+                reinterpret_cast<uint4*>(hist)[i] = make_uint4(0u, 0u, 0u, 0u);
+            __syncthreads();
+            const float ca = cosf(angle), sa = sinf(angle);
+            unsigned* hw = hist + (wave >> 1) * kHistWords;
+            const int i0 = (lane * 4 + wave) * per;
+            const int iend = min(i0 + per, nsamp);
+            for (int i = i0; i < iend; i++) {
+                const int cy = i / side, cx = i - cy * side;
+                const float py = (float)(cy - R), px = (float)(cx - R);
+                const float rx = ca * px + sa * py;
+                const float ry = -sa * px + ca * py;
+                const float xb = rx / G.bw + 4.0f / 2.0f - 0.5f;
+                const float yb = ry / G.bw + 4.0f / 2.0f - 0.5f;
+                if (!(xb > -1.0f && xb < 4.0f && yb > -1.0f && yb < 4.0f)) continue;
+                const float xs = G.xo + px, ys = G.yo + py;
+                if (xs < 0.0f || xs >= wm1 || ys < 0.0f || ys >= hm1) continue;
+                const float gx = bilinear(G.g, G.pitch, G.w, G.h, xs + 1.0f, ys, q) -
+                                 bilinear(G.g, G.pitch, G.w, G.h, xs - 1.0f, ys, q);
+                const float gy = bilinear(G.g, G.pitch, G.w, G.h, xs, ys + 1.0f, q) -
+                                 bilinear(G.g, G.pitch, G.w, G.h, xs, ys - 1.0f, q);
+                const float wt = expf(-(px * px + py * py) / G.denom);
+                const float mag = sqrtf(gx * gx + gy * gy);
+                const float pa = atan2f(gy, gx);
+                const float ar = rem_euclid(pa - angle, kTwoPi);
+                desc_scatter(hw, xb, yb, ar * 8.0f / kTwoPi, mag * wt * fix);
+            }
+            __syncthreads();
+            if (tid < 64)
+                desc_finish(hist, fix, tid, G.base, angle, G.k0 + (unsigned)pk, kcap, kps_out,
+                            desc_out);
             __syncthreads();
         }
     }
@@ -1016,6 +1192,9 @@ void launch_descriptor(const OctaveDesc* d_oct, const ImageParams* d_params, con
                        const unsigned* d_npeaks, const float* d_angles, const unsigned* d_koff,
                        int kcap, Keypoint* d_kps, float* d_desc, hipStream_t s) {
     hipLaunchKernelGGL(k_descriptor, dim3(grid_for(cap, 1, 8192)), dim3(kThreads), 0, s, d_oct,
+                       d_params, d_cand, d_total, cap, d_ref, d_npeaks, d_angles, d_koff, kcap,
+                       d_kps, d_desc);
+    hipLaunchKernelGGL(k_descriptor_wide, dim3(grid_for(cap, 1, 1024)), dim3(kThreads), 0, s, d_oct,
                        d_params, d_cand, d_total, cap, d_ref, d_npeaks, d_angles, d_koff, kcap,
                        d_kps, d_desc);
 }
