@@ -679,6 +679,33 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
+// Inclusive prefix sum over the 64 lanes of a fully active wave with DPP:
+// shifts by 1, 2, 4, 8 inside each row of 16 lanes (lanes without a source add
+// 0), then lane 15 of rows 0 and 2 is added to rows 1 and 3, and lane 31 to
+// rows 2 and 3.
+__device__ __forceinline__ int wave_incl_scan(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);   // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);   // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);   // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);   // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
+}
+
+// Sum over the 64 lanes of a fully active wave (DPP scan, then lane 63),
+// returned to every lane; the order of the additions is fixed.
+__device__ __forceinline__ float wave_sum_dpp(float v) {
+    int x = __float_as_int(v);
+    x = __float_as_int(__int_as_float(x) + __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true)));
+    x = __float_as_int(__int_as_float(x) + __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true)));
+    x = __float_as_int(__int_as_float(x) + __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true)));
+    x = __float_as_int(__int_as_float(x) + __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true)));
+    x = __float_as_int(__int_as_float(x) + __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false)));
+    x = __float_as_int(__int_as_float(x) + __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false)));
+    return __int_as_float(__builtin_amdgcn_readlane(x, 63));
+}
+
 constexpr int kDescMaxR = 31;                   // LDS path for sigma_octave <= 3.65
 constexpr int kGS = 2 * kDescMaxR + 3;          // bilinear grid row stride (floats)
 constexpr int kPatchRows = 2 * kDescMaxR + 6;   // staged patch rows
@@ -735,7 +762,8 @@ __device__ __forceinline__ void desc_row_interval(float c, float d, float B, int
         hi = keep ? 1e6f : -1e6f;
         return;
     }
-    const float e1 = (-B - d) / c, e2 = (B - d) / c;
+    const float rc = __builtin_amdgcn_rcpf(c);  // endpoints need far less than one sample of accuracy
+    const float e1 = (-B - d) * rc, e2 = (B - d) * rc;
     lo = fminf(e1, e2);
     hi = fmaxf(e1, e2);
 }
@@ -818,12 +846,12 @@ __device__ __forceinline__ void desc_finish(const unsigned* hist, float fix, int
         v[e] = (float)((double)sum / (double)fix);
     }
     float v0 = v[0], v1 = v[1];
-    const float nrm = sqrtf(wave_sum(v0 * v0 + v1 * v1));
+    const float nrm = sqrtf(wave_sum_dpp(v0 * v0 + v1 * v1));
     if (nrm >= 1e-8f) {
         const float inv = 1.0f / nrm;
         v0 = fminf(v0 * inv, 0.2f);
         v1 = fminf(v1 * inv, 0.2f);
-        const float n2 = sqrtf(wave_sum(v0 * v0 + v1 * v1));
+        const float n2 = sqrtf(wave_sum_dpp(v0 * v0 + v1 * v1));
         if (n2 >= 1e-8f) {
             const float inv2 = 1.0f / n2;
             v0 *= inv2;
@@ -878,6 +906,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
     __shared__ float4 rrec[64];  // per row: first and last column (int bits), sin * py, cos * py
     __shared__ float rwy[64];
     __shared__ int rpre[65];
+    __shared__ float pang[kMaxPeaks], pcos[kMaxPeaks], psin[kMaxPeaks];
     uint8_t* const patch = reinterpret_cast<uint8_t*>(pu);
     unsigned* const hist = pu;
     fill_q(q);
@@ -889,6 +918,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
         if (G.np == 0 || G.R > kDescMaxR) continue;  // wider windows: k_descriptor_wide
         const int R = G.R, side = G.side, gside = 2 * R + 3;
         const float xo = G.xo, yo = G.yo, bw = G.bw, denom = G.denom;
+        if (tid < G.np) {  // the peaks' angles and their cos / sin, once per keypoint
+            const float a = angles[(size_t)c * kMaxPeaks + tid];
+            pang[tid] = a;
+            pcos[tid] = cosf(a);
+            psin[tid] = sinf(a);
+        }
         {
             const int pside = 2 * R + 6;
             const int px0 = f2i_sat(floorf(xo)) - R - 2, py0 = f2i_sat(floorf(yo)) - R - 2;
@@ -949,10 +984,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
         desc_bounds(G, bx0, bx1, by0, by1);
         const int nrows = G.so > 0.0f ? max(by1 - by0 + 1, 0) : 0;  // <= 63
         for (int pk = 0; pk < G.np; pk++) {
-            const float angle = angles[(size_t)c * kMaxPeaks + pk];
+            const float angle = pang[pk];
             for (int i = tid; i < kHistCopies * kHistWords / 4; i += kThreads)
                 reinterpret_cast<uint4*>(hist)[i] = make_uint4(0u, 0u, 0u, 0u);
-            const float ca = cosf(angle), sa = sinf(angle);
+            const float ca = pcos[pk], sa = psin[pk];
             const float inv_bw = 1.0f / bw;
             const float msa = -sa;  // -sin * px is computed as (-sin) * px, the same value
             if (tid < 64) {
@@ -971,15 +1006,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                     lo = a;
                     cnt = max(b - a + 1, 0);
                     rrec[tid] = make_float4(__int_as_float(lo), __int_as_float(lo + cnt - 1), sa * py, ca * py);
-                    rwy[tid] = expf(-(float)((cy - R) * (cy - R)) / denom) * kDescFix;
+                    rwy[tid] = exs[cy] * kDescFix;
                 }
-                int incl = cnt;
-#pragma unroll
-                for (int d = 1; d < 64; d <<= 1) {
-                    const int t = __shfl_up(incl, d, 64);
-                    if (lane >= d) incl += t;
-                }
-                rpre[tid + 1] = incl;
+                rpre[tid + 1] = wave_incl_scan(cnt);
                 if (tid == 0) rpre[0] = 0;
             }
             __syncthreads();
@@ -1005,7 +1034,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                     if (cx > cxe) {  // next non-empty row (one exists: k < T)
                         do {
                             r++;
-                        } while (rpre[r + 1] == rpre[r]);
+                        } while (r < nrows - 1 && rpre[r + 1] == rpre[r]);
                         rr = rrec[r];
                         cx = __float_as_int(rr.x);
                         cxe = __float_as_int(rr.y);
