@@ -192,7 +192,7 @@ __global__ __launch_bounds__(kThreads) void k_extrema_tile(const OctaveDesc* __r
     constexpr int NG = S + 3;  // Gaussian levels
     constexpr int RH = kTileH + 2, RW = kTileW + 2;
     __shared__ float Q[NG][RH][RW];
-    const int tile = blockIdx.x;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int j = find_by(oct, noct, tile, &OctaveDesc::row_begin);
     const OctaveDesc& O = oct[j];
     const ImageParams& P = params[O.img];

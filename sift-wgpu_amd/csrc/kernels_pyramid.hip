@@ -58,7 +58,7 @@ template <int PX>
 __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__ jobs, int njobs) {
     extern __shared__ float lds[];
     constexpr int TW = 64 * PX;
-    const int tile = blockIdx.x;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int j = find_job(jobs, njobs, tile);
     const BlurJob& J = jobs[j];
     const int t = tile - J.tile_begin;
@@ -184,7 +184,7 @@ __device__ __forceinline__ void store_row4(const BlurJob& J, int y, int x, const
 }
 
 __global__ __launch_bounds__(kThreads) void k_blur_v(const BlurJob* __restrict__ jobs, int njobs) {
-    const int tile = blockIdx.x;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int j = find_job(jobs, njobs, tile);
     const BlurJob& J = jobs[j];
     const int t = tile - J.tile_begin;
@@ -270,7 +270,7 @@ constexpr int kVnTileW = 64, kVnTileH = 16;
 __global__ __launch_bounds__(kThreads) void k_blur_v_narrow(const BlurJob* __restrict__ jobs,
                                                             int njobs) {
     extern __shared__ uint8_t vl[];
-    const int tile = blockIdx.x;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int j = find_job(jobs, njobs, tile);
     const BlurJob& J = jobs[j];
     const int t = tile - J.tile_begin;
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(kThreads) void k_blur_v_narrow(const BlurJob* __res
 // Exact thread-per-pixel passes for kernels too wide for the LDS row tile.
 __global__ __launch_bounds__(kThreads) void k_blur_h_generic(const BlurJob* __restrict__ jobs,
                                                              int njobs) {
-    const int tile = blockIdx.x;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const BlurJob& J = jobs[find_job(jobs, njobs, tile)];
     const int t = tile - J.tile_begin;
     const int x = (t % J.tiles_x) * kThreads + threadIdx.x, y = t / J.tiles_x;
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(kThreads) void k_blur_h_generic(const BlurJob* __re
 
 __global__ __launch_bounds__(kThreads) void k_blur_v_generic(const BlurJob* __restrict__ jobs,
                                                              int njobs) {
-    const int tile = blockIdx.x;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const BlurJob& J = jobs[find_job(jobs, njobs, tile)];
     const int t = tile - J.tile_begin;
     const int x = (t % J.tiles_x) * kThreads + threadIdx.x, y = t / J.tiles_x;
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(kThreads) void k_blur_v_generic(const BlurJob* __re
 // Level copy (blur sigma < 1e-4, src/sift.rs:149-150) with its (zero) DoG.
 __global__ __launch_bounds__(kThreads) void k_copy_level(const BlurJob* __restrict__ jobs,
                                                          int njobs) {
-    const int tile = blockIdx.x;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const BlurJob& J = jobs[find_job(jobs, njobs, tile)];
     const int t = tile - J.tile_begin;
     const int x = (t % J.tiles_x) * kThreads + threadIdx.x, y = t / J.tiles_x;
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(kThreads) void k_copy_level(const BlurJob* __restri
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void k_resize_v(const ResizeJob* __restrict__ jobs,
                                                        int njobs) {
-    const int tile = blockIdx.x;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const ResizeJob& J = jobs[find_job(jobs, njobs, tile)];
     const int t = tile - J.tile_begin;
     const int oy = t / J.tiles_x;
@@ -428,7 +428,7 @@ constexpr int kResizeRows = 8;                   // output rows per workgroup (s
 __global__ __launch_bounds__(kThreads) void k_resize_h(const ResizeJob* __restrict__ jobs,
                                                        int njobs) {
     __shared__ float span[kResizeRows][kResizeSpan];
-    const int tile = blockIdx.x;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const ResizeJob& J = jobs[find_job(jobs, njobs, tile)];
     const int t = tile - J.tile_begin;
     const int y0 = (t / J.tiles_x) * kResizeRows;

@@ -16,6 +16,17 @@ constexpr int kMaxPeaks = 18;
 // order-independent, hence deterministic.
 constexpr double kFixScale = 1099511627776.0;  // 2^40
 
+// The dispatcher places work-group b on XCD b % 8 (each XCD has its own L2).
+// xcd_remap(b, n) is a bijection of [0, n) that gives the work-groups of one
+// XCD consecutive logical indices, so neighbouring tiles (shared halo lines)
+// are served by one L2. (The per-keypoint kernels keep the plain order: it
+// balances keypoint sizes across the XCDs better than locality pays.)
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+    const int q = n / kNumXcd, r = n % kNumXcd, x = b % kNumXcd;
+    return x * q + min(x, r) + b / kNumXcd;
+}
+
 // One separable-blur pass over one image level (horizontal or vertical).
 struct BlurJob {
     const uint8_t* src;
