@@ -66,6 +66,25 @@ __device__ __forceinline__ float atan2_grad(float y, float x) {
     return copysignf(r, y);
 }
 
+// Gradient angle in descriptor bin units (pi/4 per bin, result in (-4, 4]):
+// the same octant reduction, with a 5-term fit of atan(a)/a scaled by 4/pi
+// (max error 1.5e-5 bin, 1.2e-5 rad; far inside the descriptor tolerance).
+__device__ __forceinline__ float atan2_bins(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float a = mn * __builtin_amdgcn_rcpf(fmaxf(mx, 1e-30f));  // 0 when x = y = 0
+    const float s = a * a;
+    float p = 0.026541974395513535f;
+    p = __builtin_fmaf(p, s, -0.10842673480510712f);
+    p = __builtin_fmaf(p, s, 0.22938744723796844f);
+    p = __builtin_fmaf(p, s, -0.4205574691295624f);
+    p = __builtin_fmaf(p, s, 1.2730693817138672f);
+    float r = a * p;
+    if (ay > ax) r = 2.0f - r;
+    if (x < 0.0f) r = 4.0f - r;
+    return copysignf(r, y);
+}
+
 __device__ __forceinline__ void fill_q(float* q) {
     for (int i = threadIdx.x; i < 256; i += blockDim.x) q[i] = __fdiv_rn((float)i, 255.0f);
 }
@@ -731,13 +750,13 @@ constexpr int kMaxCopySamples = 2048;           // per-copy sample bound that ke
 __device__ __forceinline__ void desc_scatter(unsigned* hcopy, float xb, float yb, float ab,
                                              float m) {
     const float xf = floorf(xb), yf = floorf(yb), af = floorf(ab);
-    const int ai = ((int)af) & 7, p = ai & 1;
+    const unsigned ai = ((unsigned)(int)af) & 7u;
     const float dx = xb - xf, dy = yb - yf, da = ab - af;
     // word offset of cell (xf + 1, yf + 1), xf, yf in -1..3: small exact integers
-    const int cw = (int)__builtin_fmaf(yf, 6.0f * kCellWords,
-                                       __builtin_fmaf(xf, (float)kCellWords, 7.0f * kCellWords));
-    unsigned long long* base =
-        reinterpret_cast<unsigned long long*>(hcopy + p * (kTableWords + 1) + cw + ai);
+    const unsigned cw = (unsigned)(int)__builtin_fmaf(yf, 6.0f * kCellWords,
+                                                      __builtin_fmaf(xf, (float)kCellWords, 7.0f * kCellWords));
+    unsigned long long* base = reinterpret_cast<unsigned long long*>(
+        hcopy + (cw + ai + __umul24(ai & 1u, kTableWords + 1u)));
     const float t0 = m * (1.0f - dx), t1 = m * dx;
     const float tt[4] = {t0 * (1.0f - dy), t1 * (1.0f - dy), t0 * dy, t1 * dy};
     constexpr int off[4] = {0, kCellWords / 2, 6 * kCellWords / 2, 7 * kCellWords / 2};
@@ -990,6 +1009,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
             const float ca = pcos[pk], sa = psin[pk];
             const float inv_bw = 1.0f / bw;
             const float msa = -sa;  // -sin * px is computed as (-sin) * px, the same value
+            const float abin = angle * (4.0f / kPi);  // the peak angle in bins
             if (tid < 64) {
                 // per-row sample intervals of the rotated window, prefix-summed
                 int lo = 0, cnt = 0;
@@ -1005,7 +1025,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                     const int b = min((int)floorf(hif) + R + 1, bx1);
                     lo = a;
                     cnt = max(b - a + 1, 0);
-                    rrec[tid] = make_float4(__int_as_float(lo), __int_as_float(lo + cnt - 1), sa * py, ca * py);
+                    rrec[tid] = make_float4(__int_as_float(lo), __int_as_float(lo + cnt - 1),
+                                            sa * py * inv_bw, ca * py * inv_bw);
                     rwy[tid] = exs[cy] * kDescFix;
                 }
                 rpre[tid + 1] = wave_incl_scan(cnt);
@@ -1029,8 +1050,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                 int cx = __float_as_int(rr.x) + (k - rpre[r]);
                 int cxe = __float_as_int(rr.y);
                 float wyf = rwy[r];
-                const float* grow = grid + (by0 + r + 1) * kGS + 1;
-                for (; k < kend; k++, cx++) {
+                // grid point (cx, cy) of sample (cx, cy) -- the up-left neighbour -- so that
+                // all four gradient neighbours sit at non-negative offsets
+                const float* grow = grid + (by0 + r) * kGS;
+                // rotated bin coordinates (rx / bw, ry / bw) (src/sift.rs:967-968), advanced
+                // by one sample step along the row
+                const float cai = ca * inv_bw, msai = msa * inv_bw;
+                float u = __builtin_fmaf(cai, (float)(cx - R), rr.z);
+                float v = __builtin_fmaf(msai, (float)(cx - R), rr.w);
+                for (; k < kend; k++, cx++, u += cai, v += msai) {
                     if (cx > cxe) {  // next non-empty row (one exists: k < T)
                         do {
                             r++;
@@ -1039,25 +1067,21 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                         cx = __float_as_int(rr.x);
                         cxe = __float_as_int(rr.y);
                         wyf = rwy[r];
-                        grow = grid + (by0 + r + 1) * kGS + 1;
+                        grow = grid + (by0 + r) * kGS;
+                        u = __builtin_fmaf(cai, (float)(cx - R), rr.z);
+                        v = __builtin_fmaf(msai, (float)(cx - R), rr.w);
                     }
-                    // bin coordinates rx / bw + 4/2 - 0.5 (src/sift.rs:967-977), fused
-                    const float px = (float)(cx - R);
-                    const float rx = __builtin_fmaf(ca, px, rr.z);
-                    const float ry = __builtin_fmaf(msa, px, rr.w);
-                    const float xb = __builtin_fmaf(rx, inv_bw, 1.5f);
-                    const float yb = __builtin_fmaf(ry, inv_bw, 1.5f);
-                    if (!(xb > -1.0f && xb < 4.0f && yb > -1.0f && yb < 4.0f)) continue;
-                    // grid neighbours at non-negative offsets from (cx-1, cy-1)
-                    const float* gq = grow + cx - kGS - 1;
+                    // window: bin coordinates u + 1.5, v + 1.5 in (-1, 4) (src/sift.rs:969-977)
+                    if (!(fabsf(u) < 2.5f && fabsf(v) < 2.5f)) continue;
+                    const float* gq = grow + cx;
                     const float gx = gq[kGS + 2] - gq[kGS];
                     const float gyv = gq[2 * kGS + 1] - gq[1];
                     const float mag = __builtin_amdgcn_sqrtf(__builtin_fmaf(gx, gx, gyv * gyv));
                     const float m = mag * (exs[cx] * wyf);
-                    const float pa = atan2_grad(gyv, gx);
-                    const float d = pa - angle;  // in [-2pi, 2pi): rem_euclid(d, 2pi)
-                    const float ar = d < 0.0f ? d + kTwoPi : d;
-                    desc_scatter(hw, xb, yb, ar * k8OverTwoPi, m);
+                    // relative angle rem_euclid(atan2 - angle, 2pi) in bins (src/sift.rs:1009)
+                    float ab = atan2_bins(gyv, gx) - abin;
+                    ab = ab < 0.0f ? ab + 8.0f : ab;
+                    desc_scatter(hw, u + 1.5f, v + 1.5f, ab, m);
                 }
             }
             __syncthreads();
