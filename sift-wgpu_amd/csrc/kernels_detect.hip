@@ -732,7 +732,7 @@ constexpr int kPS = 72;                         // staged patch row stride (byte
 constexpr int kCellWords = 10;                  // one histogram cell: 8 angle bins + pair padding
 constexpr int kTableWords = 36 * kCellWords;    // 6 x 6 padded cells
 constexpr int kHistWords = 2 * kTableWords;     // one copy: even-pair table + odd-pair table
-constexpr int kHistCopies = 2;                  // waves {0, 1} and {2, 3} share a copy
+constexpr int kHistCopies = 2;                  // even and odd lanes scatter into separate copies
 constexpr int kMaxCopySamples = 2048;           // per-copy sample bound that keeps u32 sums exact
 
 // Trilinear scatter of one sample (src/sift.rs:1012-1049), contribution
@@ -907,8 +907,10 @@ __device__ __forceinline__ void desc_finish(const unsigned* hist, float fix, int
 // works on an in-window sample and the lanes of a wave sit far apart in the
 // window (different histogram cells). Each sample still applies the window
 // test xb, yb in (-1, 4), then computes mag * exp(-(px^2+py^2)/(2 (6 sigma_o)^2))
-// and atan2 and scatters trilinearly into its copy of the histogram (integer
-// adds: independent of the order). Wave 0 sums the copies and normalises.
+// and atan2 and scatters trilinearly into its copy of the histogram (even and
+// odd lanes use separate copies, which halves same-address collisions of the
+// LDS adds; integer adds: independent of the order). Wave 0 sums the copies
+// and normalises.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_descriptor(
     const OctaveDesc* __restrict__ oct, const ImageParams* __restrict__ params,
@@ -1038,7 +1040,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
             int k = (lane * 4 + wave) * per;
             const int kend = min(k + per, T);
             if (k < kend) {
-                unsigned* hw = hist + (wave >> 1) * kHistWords;
+                unsigned* hw = hist + (lane & 1) * kHistWords;  // neighbouring lanes: different copies
                 int r0 = 0, r1 = nrows;  // rpre[r0] <= k < rpre[r1]
                 while (r1 - r0 > 1) {
                     const int mid = (r0 + r1) >> 1;
@@ -1125,7 +1127,7 @@ __global__ __launch_bounds__(kThreads) void k_descriptor_wide(
                 reinterpret_cast<uint4*>(hist)[i] = make_uint4(0u, 0u, 0u, 0u);
             __syncthreads();
             const float ca = cosf(angle), sa = sinf(angle);
-            unsigned* hw = hist + (wave >> 1) * kHistWords;
+            unsigned* hw = hist + (lane & 1) * kHistWords;
             const int i0 = (lane * 4 + wave) * per;
             const int iend = min(i0 + per, nsamp);
             for (int i = i0; i < iend; i++) {
