@@ -747,26 +747,43 @@ constexpr int kMaxCopySamples = 2048;           // per-copy sample bound that ke
 // (1,2), (3,4), (5,6), (7,0) are aligned too. Every bin sum of a copy stays
 // below 2^32 (at most kMaxCopySamples samples of at most sqrt(2) * fix + 1/2
 // each), so the low word never carries into the high word.
-__device__ __forceinline__ void desc_scatter(unsigned* hcopy, float xb, float yb, float ab,
-                                             float m) {
+//
+// desc_bins: the bin pair address of one sample (its spatial neighbour 0), its
+// four spatial weights t_k and its angle fraction da.
+__device__ __forceinline__ unsigned long long* desc_bins(unsigned* hcopy, float xb, float yb,
+                                                         float ab, float m, float (&tt)[4],
+                                                         float& da) {
     const float xf = floorf(xb), yf = floorf(yb), af = floorf(ab);
     const unsigned ai = ((unsigned)(int)af) & 7u;
-    const float dx = xb - xf, dy = yb - yf, da = ab - af;
-    // word offset of cell (xf + 1, yf + 1), xf, yf in -1..3: small exact integers
+    const float dx = xb - xf, dy = yb - yf;
+    da = ab - af;
     const unsigned cw = (unsigned)(int)__builtin_fmaf(yf, 6.0f * kCellWords,
                                                       __builtin_fmaf(xf, (float)kCellWords, 7.0f * kCellWords));
-    unsigned long long* base = reinterpret_cast<unsigned long long*>(
-        hcopy + (cw + ai + __umul24(ai & 1u, kTableWords + 1u)));
     const float t0 = m * (1.0f - dx), t1 = m * dx;
-    const float tt[4] = {t0 * (1.0f - dy), t1 * (1.0f - dy), t0 * dy, t1 * dy};
+    tt[0] = t0 * (1.0f - dy);
+    tt[1] = t1 * (1.0f - dy);
+    tt[2] = t0 * dy;
+    tt[3] = t1 * dy;
+    return reinterpret_cast<unsigned long long*>(hcopy + (cw + ai + __umul24(ai & 1u, kTableWords + 1u)));
+}
+
+// Adds four pairs of fixed-point sums (a[2k], a[2k+1]) at the spatial
+// neighbours of a bin pair address.
+__device__ __forceinline__ void desc_add_pairs(unsigned long long* base, const float (&a)[8]) {
     constexpr int off[4] = {0, kCellWords / 2, 6 * kCellWords / 2, 7 * kCellWords / 2};
-    const float wa0 = 1.0f - da;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const unsigned lo = (unsigned)__builtin_fmaf(tt[k], wa0, 0.5f);
-        const unsigned hi = (unsigned)__builtin_fmaf(tt[k], da, 0.5f);
-        atomicAdd(base + off[k], ((unsigned long long)hi << 32) | lo);
-    }
+    for (int k = 0; k < 4; k++)
+        atomicAdd(base + off[k], ((unsigned long long)(unsigned)(a[2 * k + 1] + 0.5f) << 32) |
+                                     (unsigned)(a[2 * k] + 0.5f));
+}
+
+__device__ __forceinline__ void desc_scatter(unsigned* hcopy, float xb, float yb, float ab,
+                                             float m) {
+    float tt[4], da;
+    unsigned long long* const base = desc_bins(hcopy, xb, yb, ab, m, tt, da);
+    const float a[8] = {tt[0] * (1.0f - da), tt[0] * da, tt[1] * (1.0f - da), tt[1] * da,
+                        tt[2] * (1.0f - da), tt[2] * da, tt[3] * (1.0f - da), tt[3] * da};
+    desc_add_pairs(base, a);
 }
 
 // Interval of px where c * px + d lies in (-B, B) in real arithmetic; the
@@ -1058,6 +1075,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                 // rotated bin coordinates (rx / bw, ry / bw) (src/sift.rs:967-968), advanced
                 // by one sample step along the row
                 const float cai = ca * inv_bw, msai = msa * inv_bw;
+                // consecutive samples of a thread often share their bin pair: their
+                // contributions are summed in registers and added to LDS once per run
+                unsigned long long* acc_base = nullptr;
+                float acc[8];
                 float u = __builtin_fmaf(cai, (float)(cx - R), rr.z);
                 float v = __builtin_fmaf(msai, (float)(cx - R), rr.w);
                 for (; k < kend; k++, cx++, u += cai, v += msai) {
@@ -1083,8 +1104,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                     // relative angle rem_euclid(atan2 - angle, 2pi) in bins (src/sift.rs:1009)
                     float ab = atan2_bins(gyv, gx) - abin;
                     ab = ab < 0.0f ? ab + 8.0f : ab;
-                    desc_scatter(hw, u + 1.5f, v + 1.5f, ab, m);
+                    float tt[4], da;
+                    unsigned long long* const base = desc_bins(hw, u + 1.5f, v + 1.5f, ab, m, tt, da);
+                    if (base != acc_base) {  // a new bin pair: add the finished sums
+                        if (acc_base) desc_add_pairs(acc_base, acc);
+                        acc_base = base;
+#pragma unroll
+                        for (int q8 = 0; q8 < 8; q8++) acc[q8] = 0.0f;
+                    }
+                    const float wa0 = 1.0f - da;
+#pragma unroll
+                    for (int q4 = 0; q4 < 4; q4++) {
+                        acc[2 * q4] = __builtin_fmaf(tt[q4], wa0, acc[2 * q4]);
+                        acc[2 * q4 + 1] = __builtin_fmaf(tt[q4], da, acc[2 * q4 + 1]);
+                    }
                 }
+                if (acc_base) desc_add_pairs(acc_base, acc);
             }
             __syncthreads();
             if (tid < 64)
