@@ -19,7 +19,6 @@ namespace {
 
 constexpr float kPi = 3.14159274101257324219f;  // std::f32::consts::PI
 constexpr float kTwoPi = 2.0f * kPi;
-constexpr float k8OverTwoPi = 8.0f / kTwoPi;
 constexpr float kDescFix = 1048576.0f;  // 2^20: u32 fixed point of the descriptor sums
 constexpr int kThreads = 256;
 constexpr int kExtChunk = 1024;  // pixels of one row per chunk in k_extrema
@@ -550,17 +549,20 @@ __global__ __launch_bounds__(kThreads) void k_refine(const OctaveDesc* __restric
 
 // ---------------------------------------------------------------------------
 // k_orient: one wave (workgroup of 64) per refined keypoint, grid-stride
-// (src/sift.rs:605-726). The weights exp(-(dx^2+dy^2)/denom) are tabulated by
-// dx^2+dy^2 (the same expf input as the reference). The 64 lanes cover the
-// (2R+1)^2 window on an 8 x 8 lattice, each stepping by 8 in x and y, and add
-// mag * weight to an LDS
-// int64 fixed-point histogram, one of 8 copies by lane (integer adds:
-// order-independent, deterministic; the copies are summed afterwards).
-// Lanes 0..35 then hold one bin each for the two
-// [1,1,1]/3 smoothing passes, the max and the peak test; a ballot ranks the
-// peaks in bin order.
+// (src/sift.rs:605-726). For R <= 16 the window's rows and columns of the
+// Gaussian level (every pixel the gradient of a sample can touch) are staged
+// in LDS as q = v/255 floats. Lane l takes samples l, l+64, ... of the
+// (2R+1)^2 window in raster order; each adds mag * exp(-(dx^2+dy^2)/denom) to
+// bin floor(theta * 36 / 2pi) of a u64 fixed-point (2^40) histogram, one of 8
+// copies by lane (neighbouring lanes, whose samples are neighbours, hit
+// different copies; integer adds: order-independent, deterministic, and as
+// fine as the f32 contributions themselves). Lanes 0..35 then hold one bin
+// each for the two [1,1,1]/3 smoothing passes, the max and the peak test; a
+// ballot ranks the peaks in bin order.
 // ---------------------------------------------------------------------------
 constexpr int kOriMaxR = 16;
+constexpr int kOriRows = 2 * kOriMaxR + 5;  // staged rows
+constexpr int kOS = 2 * kOriMaxR + 12;       // staged row stride (floats), >= rows + 6
 
 __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oct,
                                                const ImageParams* __restrict__ params,
@@ -571,8 +573,8 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                                                unsigned* __restrict__ npeaks,
                                                float* __restrict__ angles) {
     __shared__ float q[256];
-    __shared__ unsigned long long hist[8][36];  // 8 copies (lane & 7): fewer atomic conflicts
-    __shared__ float wtab[2 * kOriMaxR * kOriMaxR + 1];
+    __shared__ unsigned long long hist[8][36];  // 8 copies (lane & 7)
+    __shared__ __align__(16) float pq[kOriRows * kOS];
     fill_q(q);
     __syncthreads();
     const int lane = threadIdx.x;
@@ -598,36 +600,73 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
         const int R = f2i_sat(roundf(4.5f * so));
         const float ws = 1.5f * so;
         const float denom = 2.0f * ws * ws;
-        const bool table = R <= kOriMaxR;
+        const int side = 2 * R + 1, nsamp = side * side;
+        const int iters = (nsamp + 63) >> 6;
+        // weight * 2^40 = 2^(kq * (dx^2 + dy^2) + 40)
+        const float kq = -1.44269504088896341f / denom;
+        constexpr float lf = 40.0f;  // m = mag * weight * 2^40
+        const bool lds = R <= kOriMaxR;
         for (int i = lane; i < 8 * 36; i += 64) (&hist[0][0])[i] = 0ull;
-        if (table)
-            for (int d = lane; d <= 2 * R * R; d += 64) wtab[d] = expf(-(float)d / denom);
-        __syncthreads();
-        // lane (ly, lx) = (l / 8, l % 8) takes the samples dy = ly - R + 8i, dx = lx - R + 8j
-        for (int dy = (lane >> 3) - R; dy <= R; dy += 8) {
-            const int yi = f2i_sat(roundf(yo + (float)dy));
-            if (yi < 1 || yi >= h - 1) continue;
-            const uint8_t* grow = g + (size_t)yi * pitch;
-            for (int dx = (lane & 7) - R; dx <= R; dx += 8) {
-                const int xi = f2i_sat(roundf(xo + (float)dx));
-                if (xi < 1 || xi >= w - 1) continue;
-                const uint8_t* p = grow + xi;
-                const float gx = q[p[1]] - q[p[-1]];
-                const float gy = q[p[pitch]] - q[p[-pitch]];
-                float wt;
-                if (table) {
-                    wt = wtab[dx * dx + dy * dy];
-                } else {
-                    const float fdx = (float)dx, fdy = (float)dy;
-                    wt = expf(-(fdx * fdx + fdy * fdy) / denom);
+        // the samples' pixels (round(x_o + dx), round(y_o + dy)) and their 4-neighbours
+        // lie in columns px0 .. px0 + 2R + 4 and rows py0 .. py0 + 2R + 4
+        const int px0 = f2i_sat(roundf(xo)) - R - 2, py0 = f2i_sat(roundf(yo)) - R - 2;
+        const int px0a = px0 & ~3;
+        if (lds) {
+            const int pside = 2 * R + 5;
+            const int nw = (px0 - px0a + pside + 3) >> 2;  // <= kOS / 4
+            const float inv_nw = 1.0f / (float)nw;
+            for (int i = lane; i < nw * pside; i += 64) {
+                const int v = (int)(((float)i + 0.5f) * inv_nw), wd = i - v * nw;
+                const int yy = py0 + v, xw = px0a + 4 * wd;
+                unsigned u = 0;
+                if (yy >= 0 && yy < h) {
+                    const uint8_t* row = g + (size_t)yy * pitch;
+                    if (xw >= 0 && xw + 4 <= pitch) {
+                        u = *reinterpret_cast<const unsigned*>(row + xw);
+                    } else {
+#pragma unroll
+                        for (int b = 0; b < 4; b++)
+                            if (xw + b >= 0 && xw + b < w) u |= (unsigned)row[xw + b] << (8 * b);
+                    }
                 }
-                const float mag = sqrtf(gx * gx + gy * gy);
-                const float ang = atan2_grad(gy, gx);
-                const float an = ang < 0.0f ? ang + kTwoPi : ang;
-                const float bf = an * 36.0f / kTwoPi;
-                const int bin = ((int)floorf(bf)) % 36;
-                atomicAdd(&hist[lane & 7][bin], (unsigned long long)(double)(mag * wt * (float)kFixScale));
+                *reinterpret_cast<float4*>(&pq[v * kOS + 4 * wd]) =
+                    make_float4(q[u & 255u], q[(u >> 8) & 255u], q[(u >> 16) & 255u], q[u >> 24]);
             }
+        }
+        __syncthreads();
+        const float inv_side = 1.0f / (float)side;
+        for (int it = 0; it < iters; it++) {
+            const int i = lane + 64 * it;
+            if (i >= nsamp) break;
+            const int iy = (int)(((float)i + 0.5f) * inv_side);
+            const int dy = iy - R, dx = i - iy * side - R;
+            const int yi = f2i_sat(roundf(yo + (float)dy));
+            const int xi = f2i_sat(roundf(xo + (float)dx));
+            if (yi < 1 || yi >= h - 1 || xi < 1 || xi >= w - 1) continue;
+            float gx, gy;
+            if (lds) {
+                const float* pp = &pq[(yi - py0) * kOS + (xi - px0a)];
+                gx = pp[1] - pp[-1];
+                gy = pp[kOS] - pp[-kOS];
+            } else {
+                const uint8_t* p = g + (size_t)yi * pitch + xi;
+                gx = q[p[1]] - q[p[-1]];
+                gy = q[p[pitch]] - q[p[-pitch]];
+            }
+            const float fdx = (float)dx, fdy = (float)dy;
+            const float m = __builtin_amdgcn_sqrtf(__builtin_fmaf(gx, gx, gy * gy)) *
+                            __builtin_amdgcn_exp2f(__builtin_fmaf(kq, __builtin_fmaf(fdx, fdx, fdy * fdy), lf));
+            const float ang = atan2_grad(gy, gx);
+            const float an = ang < 0.0f ? ang + kTwoPi : ang;
+            // (an * 36) / 2pi with the reference's two roundings: gradients along the
+            // axes land exactly on bin edges, where the rounding decides the bin
+            int bin = (int)floorf(an * 36.0f / kTwoPi);
+            bin = bin >= 36 ? bin - 36 : bin;
+            // m * 2^40 as a u64: the high word is exact (power-of-two scaling and
+            // floor), the remainder x - hi * 2^32 is exact (Sterbenz), both truncate
+            const float hi = floorf(m * 2.3283064365386963e-10f);
+            const float lo = __builtin_fmaf(-hi, 4294967296.0f, m);
+            atomicAdd(&hist[lane & 7][bin], ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
         }
         __syncthreads();
         float hv = 0.0f;
@@ -635,7 +674,7 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
             unsigned long long sum = 0;
 #pragma unroll
             for (int k = 0; k < 8; k++) sum += hist[k][lane];
-            hv = (float)((double)sum * (1.0 / kFixScale));
+            hv = (float)((double)sum * (1.0 / 1099511627776.0));
         }
         for (int it = 0; it < 2; it++) {
             const float pv = __shfl(hv, lane < 36 ? (lane + 35) % 36 : lane, 64);

@@ -18,7 +18,6 @@ constexpr int kThreads = 256;
 constexpr int kHTileH = 4;    // one row per wave
 constexpr int kVTileW = 256;  // 64 lanes x 4 px
 constexpr int kVTileH = 32;   // 4 waves x 8 rows
-constexpr float kAbsMargin = 6.2e-5f;
 
 template <typename Job>
 __device__ __forceinline__ int find_job(const Job* __restrict__ jobs, int njobs, int tile) {

@@ -12,9 +12,6 @@ constexpr int kTapBlock = 8;
 // Max orientation peaks per keypoint: strict local maxima of a circular 36-bin
 // histogram (src/sift.rs:691-719) are at most 18.
 constexpr int kMaxPeaks = 18;
-// Fixed-point scale of the histogram accumulators (int64 LDS atomics): sums are
-// order-independent, hence deterministic.
-constexpr double kFixScale = 1099511627776.0;  // 2^40
 
 // The dispatcher places work-group b on XCD b % 8 (each XCD has its own L2).
 // xcd_remap(b, n) is a bijection of [0, n) that gives the work-groups of one
