@@ -206,6 +206,26 @@ const LanczosRef& Context::lanczos_for(int in, int out) {
 }
 
 // ---------------------------------------------------------------------------
+// Graph cache
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kMaxGraphs = 32;  // per context
+
+bool graphs_enabled_by_env() {
+    static const bool on = [] {
+        const char* e = getenv("SIFT_HIP_GRAPHS");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+}  // namespace
+
+void Context::clear_graphs() {
+    for (GraphEntry& e : graphs) (void)hipGraphExecDestroy(e.exec);
+    graphs.clear();
+}
+
+// ---------------------------------------------------------------------------
 // Arena: one device allocation, carved per call; grows on demand.
 // ---------------------------------------------------------------------------
 int Context::ensure_arena(size_t bytes) {
@@ -213,6 +233,7 @@ int Context::ensure_arena(size_t bytes) {
     size_t ncap = std::max(bytes + bytes / 4, (size_t)64 << 20);
     if (arena) {
         HIP_TRY(hipStreamSynchronize(stream));
+        clear_graphs();  // they address the old arena
         (void)hipFree(arena);
         arena = nullptr;
         arena_cap = 0;
@@ -246,6 +267,7 @@ int Context::ensure_pinned_out(size_t bytes) {
 void Context::release() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
+    clear_graphs();
     prof_resolve();
     for (hipEvent_t e : prof_events) (void)hipEventDestroy(e);
     prof_events.clear();
@@ -725,34 +747,6 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         }
     }
     char* D = A + blob_dev_off;
-
-    // --- pyramid ---
-    for (const LaunchRec& L : launches) {
-        hipEvent_t ev = nullptr;
-        prof_begin(L.kind, L.bytes, &ev);
-        switch (L.kind) {
-            case K_BLUR_H:
-            case K_BLUR_H2:
-                launch_blur_h((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.max_pad, L.variant, st);
-                break;
-            case K_BLUR_V: launch_blur_v((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, 0, st); break;
-            case K_BLUR_V_NARROW:
-                launch_blur_v((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.variant, st);
-                break;
-            case K_BLUR_V_GENERIC:
-                launch_blur_v_generic((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st);
-                break;
-            case K_COPY: launch_copy_level((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
-            case K_BLUR_H_GENERIC:
-                launch_blur_h_generic((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st);
-                break;
-            case K_RESIZE_V: launch_resize_v((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
-            case K_RESIZE_H: launch_resize_h((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
-        }
-        prof_end(L.kind, L.bytes, ev);
-    }
-    HIP_TRY(hipGetLastError());
-
     const OctaveDesc* d_oct = (const OctaveDesc*)(D + oct_blob);
     const ImageParams* d_par = (const ImageParams*)(D + par_blob);
     unsigned* d_counts = (unsigned*)AP(counts_off);
@@ -768,7 +762,37 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     unsigned* d_sum = (unsigned*)AP(summary_off);
     unsigned* d_total = d_offs + nslots;
 
-    if (!B.pyramid_only) {
+    // Every launch of the batch, in order. Kernel arguments are arena addresses
+    // and scalars of this layout; the per-call data (images, taps) is read from
+    // the job blob uploaded above.
+    auto issue = [&](hipStream_t st) -> int {
+        // --- pyramid ---
+        for (const LaunchRec& L : launches) {
+            hipEvent_t ev = nullptr;
+            prof_begin(L.kind, L.bytes, &ev);
+            switch (L.kind) {
+                case K_BLUR_H:
+                case K_BLUR_H2:
+                    launch_blur_h((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.max_pad, L.variant, st);
+                    break;
+                case K_BLUR_V: launch_blur_v((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, 0, st); break;
+                case K_BLUR_V_NARROW:
+                    launch_blur_v((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.variant, st);
+                    break;
+                case K_BLUR_V_GENERIC:
+                    launch_blur_v_generic((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st);
+                    break;
+                case K_COPY: launch_copy_level((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
+                case K_BLUR_H_GENERIC:
+                    launch_blur_h_generic((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st);
+                    break;
+                case K_RESIZE_V: launch_resize_v((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
+                case K_RESIZE_H: launch_resize_h((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
+            }
+            prof_end(L.kind, L.bytes, ev);
+        }
+        HIP_TRY(hipGetLastError());
+        if (B.pyramid_only) return SIFT_OK;
         // --- detection ---
         double ext_bytes = 0.0;
         for (const auto& d : octs)
@@ -806,6 +830,59 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         launch_finalize(d_offs, (const int*)(D + isb_blob), n, d_koff, d_total, cap, d_sum, st);
         prof_end(K_FINALIZE, 0.0, ev);
         HIP_TRY(hipGetLastError());
+        return SIFT_OK;
+    };
+
+    if (!use_graphs || profiling || !graphs_enabled_by_env()) {
+        rc = issue(st);
+        if (rc) return rc;
+    } else {
+        // everything the launch sequence depends on
+        std::vector<uint64_t> key = {(uint64_t)(uintptr_t)A, blob_dev_off, (uint64_t)B.pyramid_only,
+                                     (uint64_t)B.want_stages, (uint64_t)tile_ext, (uint64_t)S0,
+                                     (uint64_t)noct, (uint64_t)total_tiles, (uint64_t)total_rows,
+                                     (uint64_t)nslots, (uint64_t)cap, (uint64_t)kcap, (uint64_t)n,
+                                     oct_blob, par_blob, isb_blob, bits_off, counts_off, offs_off,
+                                     scantmp1_off, cand_off, candresp_off, ref_off, valid_off, npk_off,
+                                     ang_off, koff_off, scantmp2_off, kps_off, desc_off, summary_off};
+        for (const LaunchRec& L : launches) {
+            key.push_back((uint64_t)L.kind << 32 | (uint32_t)L.njobs);
+            key.push_back(L.job_off);
+            key.push_back((uint64_t)(uint32_t)L.tiles << 32 | (uint32_t)L.max_pad);
+            key.push_back((uint64_t)(uint32_t)L.variant);
+        }
+        GraphEntry* hit = nullptr;
+        for (GraphEntry& e : graphs)
+            if (e.key == key) {
+                hit = &e;
+                break;
+            }
+        if (!hit) {
+            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            const int irc = issue(st);
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(st, &g);
+            if (irc || ce != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                if (irc) return irc;
+                return fail(SIFT_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(ce));
+            }
+            hipGraphExec_t ex = nullptr;
+            const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) return fail(SIFT_ERR_DEVICE, std::string("graph instantiation failed: ") + hipGetErrorString(ie));
+            if (graphs.size() >= kMaxGraphs) {  // evict the least recently used
+                size_t lru = 0;
+                for (size_t i = 1; i < graphs.size(); i++)
+                    if (graphs[i].used < graphs[lru].used) lru = i;
+                (void)hipGraphExecDestroy(graphs[lru].exec);
+                graphs.erase(graphs.begin() + lru);
+            }
+            graphs.push_back(GraphEntry{std::move(key), ex, 0});
+            hit = &graphs.back();
+        }
+        hit->used = ++graph_clock;
+        HIP_TRY(hipGraphLaunch(hit->exec, st));
     }
 
     if (outs) {

@@ -96,6 +96,13 @@ struct ProfStat {
     double total_ms = 0.0, bytes = 0.0;
 };
 
+// A captured launch sequence of one batch layout (see Context::run).
+struct GraphEntry {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec = nullptr;
+    uint64_t used = 0;
+};
+
 struct Context {
     int device = 0;
     sift_params_t params{};
@@ -115,6 +122,13 @@ struct Context {
     size_t cand_cap = 0, kp_cap = 0;
 
     Server* server = nullptr;  // owned; created by sift_hip_server_start
+
+    // Launch sequences replayed as HIP graphs, keyed by everything the kernel
+    // arguments depend on (arena base, layout offsets, launch shapes, capacities).
+    bool use_graphs = true;  // SIFT_HIP_GRAPHS=0 in the environment turns them off
+    std::vector<GraphEntry> graphs;
+    uint64_t graph_clock = 0;
+    void clear_graphs();
 
     bool profiling = false;
     std::vector<ProfPending> prof_pending;
