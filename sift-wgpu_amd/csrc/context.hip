@@ -223,6 +223,7 @@ bool graphs_enabled_by_env() {
 void Context::clear_graphs() {
     for (GraphEntry& e : graphs) (void)hipGraphExecDestroy(e.exec);
     graphs.clear();
+    seen_once.clear();
 }
 
 // ---------------------------------------------------------------------------
@@ -857,32 +858,51 @@ int Context::run(const Batch& B, RunOutputs* outs) {
                 hit = &e;
                 break;
             }
+        // a layout is captured the second time it is seen: one-off layouts (mixed
+        // serving batches) run eagerly instead of paying for a capture
+        bool seen = hit != nullptr;
         if (!hit) {
-            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-            const int irc = issue(st);
-            hipGraph_t g = nullptr;
-            const hipError_t ce = hipStreamEndCapture(st, &g);
-            if (irc || ce != hipSuccess) {
-                if (g) (void)hipGraphDestroy(g);
-                if (irc) return irc;
-                return fail(SIFT_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(ce));
+            uint64_t hsh = 1469598103934665603ull;  // FNV-1a over the key words
+            for (uint64_t v : key) hsh = (hsh ^ v) * 1099511628211ull;
+            for (uint64_t v : seen_once)
+                if (v == hsh) seen = true;
+            if (!seen) {
+                if (seen_once.size() >= 4 * kMaxGraphs) seen_once.erase(seen_once.begin());
+                seen_once.push_back(hsh);
             }
-            hipGraphExec_t ex = nullptr;
-            const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (ie != hipSuccess) return fail(SIFT_ERR_DEVICE, std::string("graph instantiation failed: ") + hipGetErrorString(ie));
-            if (graphs.size() >= kMaxGraphs) {  // evict the least recently used
-                size_t lru = 0;
-                for (size_t i = 1; i < graphs.size(); i++)
-                    if (graphs[i].used < graphs[lru].used) lru = i;
-                (void)hipGraphExecDestroy(graphs[lru].exec);
-                graphs.erase(graphs.begin() + lru);
-            }
-            graphs.push_back(GraphEntry{std::move(key), ex, 0});
-            hit = &graphs.back();
         }
-        hit->used = ++graph_clock;
-        HIP_TRY(hipGraphLaunch(hit->exec, st));
+        if (!seen) {
+            rc = issue(st);
+            if (rc) return rc;
+        } else {
+            if (!hit) {
+                HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+                const int irc = issue(st);
+                hipGraph_t g = nullptr;
+                const hipError_t ce = hipStreamEndCapture(st, &g);
+                if (irc || ce != hipSuccess) {
+                    if (g) (void)hipGraphDestroy(g);
+                    if (irc) return irc;
+                    return fail(SIFT_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(ce));
+                }
+                hipGraphExec_t ex = nullptr;
+                const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ie != hipSuccess)
+                    return fail(SIFT_ERR_DEVICE, std::string("graph instantiation failed: ") + hipGetErrorString(ie));
+                if (graphs.size() >= kMaxGraphs) {  // evict the least recently used
+                    size_t lru = 0;
+                    for (size_t i = 1; i < graphs.size(); i++)
+                        if (graphs[i].used < graphs[lru].used) lru = i;
+                    (void)hipGraphExecDestroy(graphs[lru].exec);
+                    graphs.erase(graphs.begin() + lru);
+                }
+                graphs.push_back(GraphEntry{std::move(key), ex, 0});
+                hit = &graphs.back();
+            }
+            hit->used = ++graph_clock;
+            HIP_TRY(hipGraphLaunch(hit->exec, st));
+        }
     }
 
     if (outs) {

@@ -31,6 +31,11 @@ double now_ms() {
 
 size_t round_up_sz(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+constexpr int kDefaultLanes = 4;
+// Pixels of one device batch (about two 1080p frames); a larger first frame
+// still runs alone.
+constexpr size_t kMaxBatchPixels = (size_t)4200000;
+
 }  // namespace
 
 // Device bytes one frame needs in a batch (pyramid, temporaries, detection
@@ -89,9 +94,14 @@ void Server::worker(Context* lane) {
             std::unique_lock<std::mutex> lk(mu);
             for (;;) {
                 if (stopping && pending.empty()) return;
+                size_t batch_px = 0;
                 while (!pending.empty() && batch.size() < cfg.max_batch) {
                     Request& r = pending.front();
                     if (bytes_in_flight + r.bytes > cfg.hbm_budget_bytes) break;
+                    // keep batches small enough that a backlog spreads over the lanes
+                    const size_t px = (size_t)r.w * r.h;
+                    if (!batch.empty() && batch_px + px > kMaxBatchPixels) break;
+                    batch_px += px;
                     bytes_in_flight += r.bytes;
                     batch_bytes += r.bytes;
                     batch.push_back(std::move(r));
@@ -156,7 +166,11 @@ int sift_hip_server_start(sift_hip_ctx* ctx, const sift_server_config_t* cfg) {
     if (S->cfg.max_batch == 0) S->cfg.max_batch = 64;
     if (S->cfg.max_queue == 0) S->cfg.max_queue = 4096;
     if (S->cfg.hbm_budget_bytes == 0) S->cfg.hbm_budget_bytes = (size_t)32 << 30;
-    for (int i = 0; i < 2; i++) {
+    // Lanes: independent streams whose batches overlap on the device; small
+    // frames leave most of the GPU idle, so several lanes raise the served rate.
+    int nlanes = kDefaultLanes;
+    if (const char* e = getenv("SIFT_HIP_LANES")) nlanes = std::min(8, std::max(1, atoi(e)));
+    for (int i = 0; i < nlanes; i++) {
         Context* L = new (std::nothrow) Context();
         if (!L) break;
         L->device = C->device;

@@ -127,6 +127,7 @@ struct Context {
     // arguments depend on (arena base, layout offsets, launch shapes, capacities).
     bool use_graphs = true;  // SIFT_HIP_GRAPHS=0 in the environment turns them off
     std::vector<GraphEntry> graphs;
+    std::vector<uint64_t> seen_once;  // hashes of keys run once, not captured yet
     uint64_t graph_clock = 0;
     void clear_graphs();
 
