@@ -330,17 +330,19 @@ struct LaunchRec {
 enum KernelId {
     K_BLUR_H = 0, K_BLUR_V, K_COPY, K_BLUR_H_GENERIC, K_RESIZE_V, K_RESIZE_H, K_EXTREMA, K_SCAN,
     K_COMPACT, K_REFINE, K_ORIENT, K_DESCRIPTOR, K_FINALIZE, K_BLUR_V_NARROW, K_BLUR_V_GENERIC,
-    K_BLUR_H2, K_COUNT
+    K_BLUR_H2, K_BLUR_H_SMALL, K_BLUR_V_SMALL, K_COUNT
 };
 // Names as rocprofv3 reports the kernels (template arguments included).
 const char* const kKernelNames[K_COUNT] = {
     "k_blur_h<8>", "k_blur_v", "k_copy_level", "k_blur_h_generic", "k_resize_v", "k_resize_h",
     "k_extrema_tile", "k_scan", "k_compact", "k_refine", "k_orient", "k_descriptor",
-    "k_finalize", "k_blur_v_narrow", "k_blur_v_generic", "k_blur_h<2>"};
+    "k_finalize", "k_blur_v_narrow", "k_blur_v_generic", "k_blur_h<2>", "k_blur_h_small",
+    "k_blur_v_small"};
 
 double blur_job_bytes(const BlurJob& j, int kind) {
     double a = (double)j.w * j.h;
-    if ((kind == K_BLUR_V || kind == K_BLUR_V_NARROW || kind == K_BLUR_V_GENERIC) && j.dog)
+    if ((kind == K_BLUR_V || kind == K_BLUR_V_NARROW || kind == K_BLUR_V_GENERIC ||
+         kind == K_BLUR_V_SMALL) && j.dog)
         return a * (2.0 + 1.0 + 4.0);
     if (kind == K_COPY && j.dog) return a * (2.0 + 4.0);
     return a * 2.0;
@@ -527,7 +529,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         maxO = std::max(maxO, (int)shapes[i].size());
         Lmax = std::max(Lmax, (int)B.params[i].num_intervals + 3);
     }
-    const size_t max_launches = (size_t)(1 + maxO * Lmax) * 8 + (size_t)maxO * 2 + 8;
+    const size_t max_launches = (size_t)(1 + maxO * Lmax) * 10 + (size_t)maxO * 2 + 8;
     const size_t max_jobs = (size_t)n * ((1 + (size_t)maxO * Lmax) * 2 + (size_t)maxO * 2);
     const size_t blob_bound = max_launches * 256 + max_jobs * std::max(sizeof(BlurJob), sizeof(ResizeJob)) +
                               octs.size() * sizeof(OctaveDesc) + (size_t)n * sizeof(ImageParams) +
@@ -539,8 +541,8 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     char* T = tables.dev;
     auto AP = [&](size_t off) { return A + off; };
 
-    std::vector<BlurJob> hj, hj2, hgj, vj, vnj, vgj, cj;
-    int vn_rows = 0;
+    std::vector<BlurJob> hj, hj2, hgj, hsj, vj, vnj, vgj, vsj, cj;
+    int vn_rows = 0, hs_taps4 = 0, vs_taps4 = 0, vs_rows = 0;
 
     auto job_bytes = [](const auto& j, int kind) -> double {
         using J = std::decay_t<decltype(j)>;
@@ -591,7 +593,15 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         H.dst_pitch = tmp_pitch;
         const int px = w >= 384 ? 8 : 2;
         const int hrow = 64 * px + R.npad + 8;
-        if ((hrow + (hrow >> 5) + 1) * 4 * (int)sizeof(float) <= 65536) {
+        const int n4 = (R.ntaps + 3) & ~3, r = R.ntaps / 2;
+        // small levels and long kernels on narrow levels: one output per lane
+        const bool small_level = (long long)w * h <= 65536 || r >= 24;
+        if (w < 384 && n4 <= kSmallMaxTaps4 && small_level) {
+            H.tiles_x = (w + 63) / 64;
+            H.tiles = H.tiles_x * ((h + 3) / 4);
+            hsj.push_back(H);
+            hs_taps4 = std::max(hs_taps4, n4);
+        } else if ((hrow + (hrow >> 5) + 1) * 4 * (int)sizeof(float) <= 65536) {
             H.tiles_x = (w + 64 * px - 1) / (64 * px);
             H.tiles = H.tiles_x * ((h + 3) / 4);
             (px == 8 ? hj : hj2).push_back(H);
@@ -608,9 +618,14 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         V.dog = dog;
         V.prev = prev;
         V.dog_pitch = dog_pitch;
-        const int r = R.ntaps / 2;
         const int narrow_rows = std::min(h, 16 + 2 * r);
-        if (w >= 192 && R.ntaps <= 64) {
+        if (h <= kSmallMaxRows && n4 <= kSmallMaxTaps4 && (small_level || w < 192 || R.ntaps > 64)) {
+            V.tiles_x = (w + 63) / 64;
+            V.tiles = V.tiles_x * ((h + 3) / 4);
+            vsj.push_back(V);
+            vs_taps4 = std::max(vs_taps4, n4);
+            vs_rows = std::max(vs_rows, std::min(h, 4 + 2 * r));
+        } else if (w >= 192 && R.ntaps <= 64) {
             V.tiles_x = (w + 255) / 256;
             V.tiles = V.tiles_x * ((h + 31) / 32);
             vj.push_back(V);
@@ -633,12 +648,15 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         push_launch(K_BLUR_H, hj, maxpad, 8);
         push_launch(K_BLUR_H2, hj2, maxpad2, 2);
         push_launch(K_BLUR_H_GENERIC, hgj, 0, 0);
+        push_launch(K_BLUR_H_SMALL, hsj, hs_taps4, 0);
         push_launch(K_BLUR_V, vj, 0, 0);
         push_launch(K_BLUR_V_NARROW, vnj, 0, vn_rows);
         push_launch(K_BLUR_V_GENERIC, vgj, 0, 0);
+        push_launch(K_BLUR_V_SMALL, vsj, vs_taps4, vs_rows);
         push_launch(K_COPY, cj, 0, 0);
-        hj.clear(); hj2.clear(); hgj.clear(); vj.clear(); vnj.clear(); vgj.clear(); cj.clear();
-        vn_rows = 0;
+        hj.clear(); hj2.clear(); hgj.clear(); hsj.clear(); vj.clear(); vnj.clear(); vgj.clear(); vsj.clear();
+        cj.clear();
+        vn_rows = hs_taps4 = vs_taps4 = vs_rows = 0;
     };
 
     // initial blur into octave 0 level 0
@@ -786,6 +804,13 @@ int Context::run(const Batch& B, RunOutputs* outs) {
                 case K_COPY: launch_copy_level((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
                 case K_BLUR_H_GENERIC:
                     launch_blur_h_generic((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st);
+                    break;
+                case K_BLUR_H_SMALL:
+                    launch_blur_h_small((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.max_pad, st);
+                    break;
+                case K_BLUR_V_SMALL:
+                    launch_blur_v_small((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.max_pad,
+                                        L.variant, st);
                     break;
                 case K_RESIZE_V: launch_resize_v((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
                 case K_RESIZE_H: launch_resize_h((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;

@@ -320,6 +320,101 @@ __global__ __launch_bounds__(kThreads) void k_blur_v_narrow(const BlurJob* __res
     }
 }
 
+// ---------------------------------------------------------------------------
+// Small levels (the deep octaves: a few thousand pixels, blur radii up to
+// several hundred): one output per lane, so each wave's serial tap chain is as
+// short as it can be and the few tiles still spread over many waves.
+// Tile = 64 columns x 4 rows, wave w computes row y0+w, lane l column x0+l.
+// The taps (zero-padded to a multiple of 4) sit in LDS and are read four at a
+// time by all lanes at once. Every output keeps the reference's tap order and
+// separate f32 multiply and add.
+// ---------------------------------------------------------------------------
+constexpr int kSTileW = 64, kSTileH = 4;
+
+// Horizontal: the clamped row segments [x0-r, x0+64+n4-r) of the tile's 4 rows
+// are staged as f32 (segment stride seg_stride >= 64 + n4).
+__global__ __launch_bounds__(kThreads) void k_blur_h_small(const BlurJob* __restrict__ jobs,
+                                                           int njobs, int seg_stride) {
+    extern __shared__ float sm[];
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const BlurJob& J = jobs[find_job(jobs, njobs, tile)];
+    const int t = tile - J.tile_begin;
+    const int x0 = (t % J.tiles_x) * kSTileW, y0 = (t / J.tiles_x) * kSTileH;
+    const int w = J.w, h = J.h, n = J.ntaps, r = n >> 1;
+    const int n4 = (n + 3) & ~3;
+    float* tp = sm;
+    float* seg = sm + n4;
+    for (int i = threadIdx.x; i < n4; i += kThreads) tp[i] = i < n ? J.taps[i] : 0.0f;
+    const int segw = kSTileW + n4;
+    const float inv_segw = 1.0f / (float)segw;
+    for (int i = threadIdx.x; i < kSTileH * segw; i += kThreads) {
+        const int rr = (int)(((float)i + 0.5f) * inv_segw), c = i - rr * segw;
+        const int y = min(y0 + rr, h - 1);
+        const int x = min(max(x0 - r + c, 0), w - 1);
+        seg[rr * seg_stride + c] = (float)J.src[(size_t)y * J.src_pitch + x];
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int y = y0 + wave, x = x0 + lane;
+    if (y >= h || x >= w) return;
+    const float* sg = seg + wave * seg_stride + lane;
+    float acc = 0.0f;
+#pragma unroll 2
+    for (int k = 0; k < n4; k += 4) {
+        const float4 t4 = *reinterpret_cast<const float4*>(tp + k);
+        acc = mul_add_exact(acc, sg[k], t4.x);
+        acc = mul_add_exact(acc, sg[k + 1], t4.y);
+        acc = mul_add_exact(acc, sg[k + 2], t4.z);
+        acc = mul_add_exact(acc, sg[k + 3], t4.w);
+    }
+    J.dst[(size_t)y * J.dst_pitch + x] = (uint8_t)u8_trunc(acc);
+}
+
+// Vertical: the tile's distinct input rows [max(0, y0-r), min(h, y0+4+r)) are
+// staged as f32 (at most h rows). For output row y, taps k < r-y read row 0 and
+// taps k >= h+r-y read row h-1 (clamp to edge): those runs need no loads and
+// their bounds are the same for the whole wave.
+__global__ __launch_bounds__(kThreads) void k_blur_v_small(const BlurJob* __restrict__ jobs,
+                                                           int njobs, int max_taps4) {
+    extern __shared__ float sm[];
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const BlurJob& J = jobs[find_job(jobs, njobs, tile)];
+    const int t = tile - J.tile_begin;
+    const int x0 = (t % J.tiles_x) * kSTileW, y0 = (t / J.tiles_x) * kSTileH;
+    const int w = J.w, h = J.h, n = J.ntaps, r = n >> 1;
+    const int n4 = (n + 3) & ~3;
+    float* tp = sm;
+    float* col = sm + max_taps4;
+    for (int i = threadIdx.x; i < n4; i += kThreads) tp[i] = i < n ? J.taps[i] : 0.0f;
+    const int rlo = max(y0 - r, 0), rhi = min(y0 + kSTileH + r, h);
+    for (int i = threadIdx.x; i < (rhi - rlo) * kSTileW; i += kThreads) {
+        const int rr = i >> 6, c = i & 63;
+        col[i] = (float)J.src[(size_t)(rlo + rr) * J.src_pitch + min(x0 + c, w - 1)];
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int y = y0 + wave, x = x0 + lane;
+    if (y >= h || x >= w) return;
+    const int ka = min(max(r - y, 0), n);      // taps reading row 0
+    const int kc = max(min(h + r - y, n), ka);  // taps from kc on read row h-1
+    float acc = 0.0f;
+    if (ka > 0) {
+        const float p0 = col[(0 - rlo) * kSTileW + lane];
+        for (int k = 0; k < ka; k++) acc = mul_add_exact(acc, p0, tp[k]);
+    }
+    const float* cp = col + (y - r - rlo) * kSTileW + lane;
+#pragma unroll 4
+    for (int k = ka; k < kc; k++) acc = mul_add_exact(acc, cp[k * kSTileW], tp[k]);
+    if (kc < n) {
+        const float pl = col[(h - 1 - rlo) * kSTileW + lane];
+        for (int k = kc; k < n; k++) acc = mul_add_exact(acc, pl, tp[k]);
+    }
+    const unsigned v = u8_trunc(acc);
+    J.dst[(size_t)y * J.dst_pitch + x] = (uint8_t)v;
+    if (J.dog)
+        J.dog[(size_t)y * J.dog_pitch + x] = q255(v) - q255(J.prev[(size_t)y * J.dst_pitch + x]);
+}
+
 // Exact thread-per-pixel passes for kernels too wide for the LDS row tile.
 __global__ __launch_bounds__(kThreads) void k_blur_h_generic(const BlurJob* __restrict__ jobs,
                                                              int njobs) {
@@ -490,6 +585,23 @@ void launch_blur_v(const BlurJob* d_jobs, int njobs, int total_tiles, int narrow
     } else {
         hipLaunchKernelGGL(k_blur_v, dim3(total_tiles), dim3(kThreads), 0, s, d_jobs, njobs);
     }
+}
+
+void launch_blur_h_small(const BlurJob* d_jobs, int njobs, int total_tiles, int max_taps4,
+                         hipStream_t s) {
+    if (total_tiles <= 0) return;
+    const int seg_stride = kSTileW + max_taps4 + 1;
+    const size_t lds = ((size_t)max_taps4 + (size_t)kSTileH * seg_stride) * sizeof(float);
+    hipLaunchKernelGGL(k_blur_h_small, dim3(total_tiles), dim3(kThreads), lds, s, d_jobs, njobs,
+                       seg_stride);
+}
+
+void launch_blur_v_small(const BlurJob* d_jobs, int njobs, int total_tiles, int max_taps4,
+                         int max_rows, hipStream_t s) {
+    if (total_tiles <= 0) return;
+    const size_t lds = ((size_t)max_taps4 + (size_t)max_rows * kSTileW) * sizeof(float);
+    hipLaunchKernelGGL(k_blur_v_small, dim3(total_tiles), dim3(kThreads), lds, s, d_jobs, njobs,
+                       max_taps4);
 }
 
 void launch_blur_h_generic(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s) {

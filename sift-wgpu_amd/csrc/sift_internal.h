@@ -102,6 +102,13 @@ void launch_blur_v(const BlurJob* d_jobs, int njobs, int total_tiles, int narrow
 void launch_resize_v(const ResizeJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
 void launch_resize_h(const ResizeJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
 void launch_copy_level(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
+// Small levels: one output per lane (tiles of 64 x 4). max_taps4 = the largest
+// tap count of the launch rounded up to 4; max_rows = the largest number of
+// distinct input rows a vertical tile stages (min(h, 4 + 2r)).
+void launch_blur_h_small(const BlurJob* d_jobs, int njobs, int total_tiles, int max_taps4,
+                         hipStream_t s);
+void launch_blur_v_small(const BlurJob* d_jobs, int njobs, int total_tiles, int max_taps4,
+                         int max_rows, hipStream_t s);
 void launch_blur_h_generic(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
 void launch_blur_v_generic(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
 
@@ -135,6 +142,9 @@ void launch_finalize(const unsigned* d_slot_offsets, const int* d_img_slot_begin
                      unsigned* d_summary, hipStream_t s);
 // Max padded tap count handled by the LDS row-tile horizontal pass.
 constexpr int kMaxHPad = 3576;
+// Small-level kernels: taps (rounded up to 4) and staged rows they accept.
+constexpr int kSmallMaxTaps4 = 2800;
+constexpr int kSmallMaxRows = 128;
 constexpr int kScanTileElems = 4096;
 
 }  // namespace sh
