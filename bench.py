@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--no-profile", action="store_true", help="skip the HIP-event roofline")
     ap.add_argument("--mixed-batch", type=int, default=64,
                     help="frames in the mixed-size batch leg (0 disables it)")
-    ap.add_argument("--serve-seconds", type=float, default=5.0,
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the served open-loop stream (0 disables it)")
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="requests/s of the stream")
     ap.add_argument("--traffic", default=os.path.join(REPO, "profiles", "traffic_1080p_b32.txt"),

@@ -515,6 +515,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     const size_t kps_off = C.take((size_t)kcap * sizeof(Keypoint));
     const size_t desc_off = C.take((size_t)kcap * 128 * 4);
     const size_t summary_off = C.take((size_t)(n + 3) * 4);
+    const size_t wide_off = C.take((size_t)cap * 4);  // keypoints for the wide-window descriptor
 
     // --- job blob (host), uploaded to the end of the arena ---
     std::vector<char> blob;
@@ -829,7 +830,8 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             launch_extrema_tile(S0, d_oct, noct, total_tiles, d_par, (unsigned long long*)AP(bits_off),
                                 d_counts, nslots, st);
         else
-            launch_extrema(d_oct, noct, total_rows, d_par, (unsigned long long*)AP(bits_off), d_counts, st);
+            launch_extrema(d_oct, noct, total_rows, d_par, (unsigned long long*)AP(bits_off), d_counts,
+                           nslots, st);
         prof_end(K_EXTREMA, ext_bytes, ev);
         prof_begin(K_SCAN, 0.0, &ev);
         launch_scan_dev(d_counts, nslots, nullptr, d_offs, (unsigned*)AP(scantmp1_off), st);
@@ -850,7 +852,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         prof_end(K_SCAN, 0.0, ev);
         prof_begin(K_DESCRIPTOR, 0.0, &ev);
         launch_descriptor(d_oct, d_par, d_cand, d_total, cap, d_ref, d_npk, d_ang, d_koff, kcap, d_kps,
-                          d_desc, st);
+                          d_desc, d_counts + nslots, (int*)AP(wide_off), st);
         prof_end(K_DESCRIPTOR, 0.0, ev);
         prof_begin(K_FINALIZE, 0.0, &ev);
         launch_finalize(d_offs, (const int*)(D + isb_blob), n, d_koff, d_total, cap, d_sum, st);
@@ -870,7 +872,8 @@ int Context::run(const Batch& B, RunOutputs* outs) {
                                      (uint64_t)nslots, (uint64_t)cap, (uint64_t)kcap, (uint64_t)n,
                                      oct_blob, par_blob, isb_blob, bits_off, counts_off, offs_off,
                                      scantmp1_off, cand_off, candresp_off, ref_off, valid_off, npk_off,
-                                     ang_off, koff_off, scantmp2_off, kps_off, desc_off, summary_off};
+                                     ang_off, koff_off, scantmp2_off, kps_off, desc_off, summary_off,
+                                     wide_off};
         for (const LaunchRec& L : launches) {
             key.push_back((uint64_t)L.kind << 32 | (uint32_t)L.njobs);
             key.push_back(L.job_off);

@@ -973,7 +973,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
     const Cand* __restrict__ cand, const unsigned* __restrict__ total, int cap,
     const Keypoint* __restrict__ ref, const unsigned* __restrict__ npeaks,
     const float* __restrict__ angles, const unsigned* __restrict__ koff, int kcap,
-    Keypoint* __restrict__ kps_out, float* __restrict__ desc_out) {
+    Keypoint* __restrict__ kps_out, float* __restrict__ desc_out, unsigned* __restrict__ wide_count,
+    int* __restrict__ wide_list) {
     static_assert(kHistCopies * kHistWords * 4 >= kPatchRows * kPS, "patch shares the histograms");
     __shared__ float q[256];
     // the u8 patch is dead once the grid is built; the histograms live after that
@@ -992,7 +993,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
     const int n = (int)min(*total, (unsigned)cap);
     for (int c = blockIdx.x; c < n; c += gridDim.x) {
         const DescGeom G = desc_geom(c, oct, params, cand, ref, npeaks, koff);
-        if (G.np == 0 || G.R > kDescMaxR) continue;  // wider windows: k_descriptor_wide
+        if (G.np == 0) continue;
+        if (G.R > kDescMaxR) {  // wider windows: listed for k_descriptor_wide
+            if (threadIdx.x == 0) wide_list[atomicAdd(wide_count, 1u)] = c;
+            continue;
+        }
         const int R = G.R, side = G.side, gside = 2 * R + 3;
         const float xo = G.xo, yo = G.yo, bw = G.bw, denom = G.denom;
         if (tid < G.np) {  // the peaks' angles and their cos / sin, once per keypoint
@@ -1181,16 +1186,19 @@ __global__ __launch_bounds__(kThreads) void k_descriptor_wide(
     const Cand* __restrict__ cand, const unsigned* __restrict__ total, int cap,
     const Keypoint* __restrict__ ref, const unsigned* __restrict__ npeaks,
     const float* __restrict__ angles, const unsigned* __restrict__ koff, int kcap,
-    Keypoint* __restrict__ kps_out, float* __restrict__ desc_out) {
+    Keypoint* __restrict__ kps_out, float* __restrict__ desc_out,
+    const unsigned* __restrict__ wide_count, const int* __restrict__ wide_list) {
     __shared__ float q[256];
     __shared__ __align__(16) unsigned hist[kHistCopies * kHistWords];
     fill_q(q);
     __syncthreads();
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int n = (int)min(*total, (unsigned)cap);
-    for (int c = blockIdx.x; c < n; c += gridDim.x) {
+    (void)total;
+    (void)cap;
+    const int nw = (int)*wide_count;  // keypoints k_descriptor listed
+    for (int i = blockIdx.x; i < nw; i += gridDim.x) {
+        const int c = wide_list[i];
         const DescGeom G = desc_geom(c, oct, params, cand, ref, npeaks, koff);
-        if (G.np == 0 || G.R <= kDescMaxR) continue;  // k_descriptor covers these
         const int R = G.R, side = G.side, nsamp = side * side;
         const float wm1 = (float)(G.w - 1), hm1 = (float)(G.h - 1);
         const int per = (nsamp + kThreads - 1) / kThreads;
@@ -1253,7 +1261,9 @@ __global__ void k_finalize(const unsigned* __restrict__ slot_offsets,
 }  // namespace
 
 void launch_extrema(const OctaveDesc* d_oct, int noct, int total_rows, const ImageParams* d_params,
-                    unsigned long long* d_bits, unsigned* d_counts, hipStream_t s) {
+                    unsigned long long* d_bits, unsigned* d_counts, int nslots, hipStream_t s) {
+    // counts[nslots] is the descriptor's wide-window keypoint count
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, d_counts + nslots, 1, 0u);
     if (total_rows <= 0) return;
     hipLaunchKernelGGL(k_extrema, dim3(total_rows), dim3(kThreads), 0, s, d_oct, noct, d_params,
                        d_bits, d_counts);
@@ -1264,8 +1274,8 @@ bool extrema_tile_supported(int S) { return S >= 1 && S <= 6; }
 void launch_extrema_tile(int S, const OctaveDesc* d_oct, int noct, int total_tiles,
                          const ImageParams* d_params, unsigned long long* d_bits, unsigned* d_counts,
                          int nslots, hipStream_t s) {
-    if (nslots > 0)
-        hipLaunchKernelGGL(k_fill_u32, dim3((nslots + 255) / 256), dim3(256), 0, s, d_counts, nslots, 0u);
+    // counts[nslots] is the descriptor's wide-window keypoint count
+    hipLaunchKernelGGL(k_fill_u32, dim3((nslots + 1 + 255) / 256), dim3(256), 0, s, d_counts, nslots + 1, 0u);
     if (total_tiles <= 0) return;
     switch (S) {
 #define SH_EXT(SS) \
@@ -1319,13 +1329,14 @@ void launch_orient(const OctaveDesc* d_oct, const ImageParams* d_params, const C
 void launch_descriptor(const OctaveDesc* d_oct, const ImageParams* d_params, const Cand* d_cand,
                        const unsigned* d_total, int cap, const Keypoint* d_ref,
                        const unsigned* d_npeaks, const float* d_angles, const unsigned* d_koff,
-                       int kcap, Keypoint* d_kps, float* d_desc, hipStream_t s) {
+                       int kcap, Keypoint* d_kps, float* d_desc, unsigned* d_wide_count,
+                       int* d_wide_list, hipStream_t s) {
     hipLaunchKernelGGL(k_descriptor, dim3(grid_for(cap, 1, 8192)), dim3(kThreads), 0, s, d_oct,
                        d_params, d_cand, d_total, cap, d_ref, d_npeaks, d_angles, d_koff, kcap,
-                       d_kps, d_desc);
-    hipLaunchKernelGGL(k_descriptor_wide, dim3(grid_for(cap, 1, 1024)), dim3(kThreads), 0, s, d_oct,
-                       d_params, d_cand, d_total, cap, d_ref, d_npeaks, d_angles, d_koff, kcap,
-                       d_kps, d_desc);
+                       d_kps, d_desc, d_wide_count, d_wide_list);
+    hipLaunchKernelGGL(k_descriptor_wide, dim3(256), dim3(kThreads), 0, s, d_oct, d_params, d_cand,
+                       d_total, cap, d_ref, d_npeaks, d_angles, d_koff, kcap, d_kps, d_desc,
+                       d_wide_count, d_wide_list);
 }
 
 void launch_finalize(const unsigned* d_slot_offsets, const int* d_img_slot_begin, int nimg,

@@ -113,7 +113,7 @@ void launch_blur_h_generic(const BlurJob* d_jobs, int njobs, int total_tiles, hi
 void launch_blur_v_generic(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
 
 void launch_extrema(const OctaveDesc* d_oct, int noct, int total_rows, const ImageParams* d_params,
-                    unsigned long long* d_bits, unsigned* d_counts, hipStream_t s);
+                    unsigned long long* d_bits, unsigned* d_counts, int nslots, hipStream_t s);
 // Tiled extremum scan (64x16 tiles) for S in [1, 6]: zeroes the row-slot
 // counts, then one workgroup per tile; OctaveDesc.row_begin holds the tile prefix.
 bool extrema_tile_supported(int S);
@@ -134,7 +134,8 @@ void launch_orient(const OctaveDesc* d_oct, const ImageParams* d_params, const C
 void launch_descriptor(const OctaveDesc* d_oct, const ImageParams* d_params, const Cand* d_cand,
                        const unsigned* d_total, int cap, const Keypoint* d_ref,
                        const unsigned* d_npeaks, const float* d_angles, const unsigned* d_koff,
-                       int kcap, Keypoint* d_kps, float* d_desc, hipStream_t s);
+                       int kcap, Keypoint* d_kps, float* d_desc, unsigned* d_wide_count,
+                       int* d_wide_list, hipStream_t s);
 // Writes summary[0] = candidates, summary[1] = keypoints, summary[2..2+nimg] =
 // per-image keypoint offsets (nimg+1 entries).
 void launch_finalize(const unsigned* d_slot_offsets, const int* d_img_slot_begin, int nimg,
