@@ -26,6 +26,7 @@ def _sift(octaves=4, contrast=0.04, S=3, edge=10.0, sigma=1.6, blur=0.5):
 CASES = {
     "c1": lambda: imgen.config_image("c1"),
     "c2": lambda: imgen.config_image("c2"),
+    "c3": lambda: imgen.config_image("c3"),  # 3840x2160, 6 octaves, contrast 0.03
     "odd_97x61": lambda: (imgen.generic_image(97, 61, seed=9), 3, 0.04),
     "wide_333x45": lambda: (imgen.generic_image(333, 45, seed=10), 4, 0.04),
 }
@@ -42,7 +43,7 @@ def test_pyramid_and_dog_bit_exact(case):
         assert np.array_equal(d_hip[o], d_ref[o]), f"dog octave {o}"
 
 
-@pytest.mark.parametrize("case", ["c1", "c2", "odd_97x61"])
+@pytest.mark.parametrize("case", ["c1", "c2", "c3", "odd_97x61"])
 def test_candidates_and_refined_bit_exact(case):
     img, octv, ct = CASES[case]()
     s, p = _sift(octv, ct)
@@ -64,7 +65,7 @@ def _parity(img, s, p):
     return kps, desc, okps, m
 
 
-@pytest.mark.parametrize("case", ["c1", "c2", "odd_97x61", "wide_333x45"])
+@pytest.mark.parametrize("case", ["c1", "c2", "c3", "odd_97x61", "wide_333x45"])
 def test_keypoints_and_descriptors_match_oracle(case):
     img, octv, ct = CASES[case]()
     s, p = _sift(octv, ct)
