@@ -151,6 +151,10 @@ typedef struct sift_kernel_stat_t {
 } sift_kernel_stat_t;
 
 int sift_hip_profile(sift_hip_ctx* ctx, int enable);
+/* Restricts the timing to launches of one kernel (its name as in
+ * sift_kernel_stat_t, e.g. "k_blur_h<8>"), so that the events add little to
+ * the timed work; NULL or "" times every kernel. Unknown name: SIFT_ERR_INVALID. */
+int sift_hip_profile_filter(sift_hip_ctx* ctx, const char* kernel);
 int sift_hip_profile_read(sift_hip_ctx* ctx, sift_kernel_stat_t* out, size_t cap, size_t* n);
 
 /* ---- Serving: continuous batching on one device -------------------------

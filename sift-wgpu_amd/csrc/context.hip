@@ -369,7 +369,7 @@ void Context::prof_begin(int kernel, double bytes, hipEvent_t* a) {
     (void)kernel;
     (void)bytes;
     *a = nullptr;
-    if (!profiling) return;
+    if (!profiling || (prof_only >= 0 && kernel != prof_only)) return;
     *a = prof_event();
     if (*a) (void)hipEventRecord(*a, stream);
 }
@@ -1265,6 +1265,20 @@ int sift_hip_profile(sift_hip_ctx* ctx, int enable) {
     C->prof_resolve();
     C->prof_stats.clear();
     C->profiling = enable != 0;
+    return SIFT_OK;
+}
+
+int sift_hip_profile_filter(sift_hip_ctx* ctx, const char* kernel) {
+    if (!ctx) return fail(SIFT_ERR_INVALID, "null context");
+    sh::Context* C = reinterpret_cast<sh::Context*>(ctx);
+    std::lock_guard<std::mutex> lk(C->mu);
+    int k = -1;
+    if (kernel && kernel[0]) {
+        for (int i = 0; i < K_COUNT; i++)
+            if (strcmp(kKernelNames[i], kernel) == 0) k = i;
+        if (k < 0) return fail(SIFT_ERR_INVALID, std::string("unknown kernel name: ") + kernel);
+    }
+    C->prof_only = k;
     return SIFT_OK;
 }
 

@@ -132,6 +132,7 @@ struct Context {
     void clear_graphs();
 
     bool profiling = false;
+    int prof_only = -1;  // kernel id whose launches are timed, -1 = all
     std::vector<ProfPending> prof_pending;
     std::vector<hipEvent_t> prof_events;  // free pool
     std::map<int, ProfStat> prof_stats;

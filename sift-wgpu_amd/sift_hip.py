@@ -76,7 +76,8 @@ EXPORTED_SYMBOLS = [
     "sift_params_default", "sift_params_new", "sift_hip_create", "sift_hip_destroy",
     "sift_hip_last_error", "sift_hip_detect", "sift_hip_detect_batch",
     "sift_hip_detect_batch_device", "sift_result_free", "sift_hip_pyramid",
-    "sift_hip_detect_stages", "sift_stages_free", "sift_hip_profile", "sift_hip_profile_read",
+    "sift_hip_detect_stages", "sift_stages_free", "sift_hip_profile", "sift_hip_profile_filter",
+    "sift_hip_profile_read",
     "sift_hip_server_start", "sift_hip_submit", "sift_hip_poll", "sift_hip_server_stop",
 ]
 
@@ -120,6 +121,8 @@ def lib():
         L.sift_stages_free.argtypes = [P(SiftStages)]
         L.sift_hip_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.sift_hip_profile.restype = ctypes.c_int
+        L.sift_hip_profile_filter.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        L.sift_hip_profile_filter.restype = ctypes.c_int
         L.sift_hip_profile_read.argtypes = [ctypes.c_void_p, P(SiftKernelStat), ctypes.c_size_t,
                                             P(ctypes.c_size_t)]
         L.sift_hip_profile_read.restype = ctypes.c_int
@@ -381,10 +384,13 @@ class Sift:
         return cand, ref, kps, desc
 
 
-def profile(enable=True, device=0):
-    """Enables (and resets) or disables per-kernel event timing on the
-    process-global context."""
-    _check(lib().sift_hip_profile(_Ctx.get(device), 1 if enable else 0), "sift_hip_profile")
+def profile(enable=True, device=0, only=None):
+    """sift_hip_profile: per-kernel HIP-event timing of the context's launches
+    (resets the counters). only: time just that kernel's launches (e.g.
+    "k_blur_h<8>"), so that the events add little to the timed work."""
+    ctx = _Ctx.get(device)
+    _check(lib().sift_hip_profile(ctx, 1 if enable else 0), "sift_hip_profile")
+    _check(lib().sift_hip_profile_filter(ctx, only.encode() if only else None), "sift_hip_profile_filter")
 
 
 def profile_read(device=0):
