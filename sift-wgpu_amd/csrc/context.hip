@@ -220,6 +220,20 @@ bool graphs_enabled_by_env() {
 }
 }  // namespace
 
+// Initial candidate capacity max(min, pixel-layers / divisor): 4096 and 128 by
+// default; SIFT_HIP_CAP_MIN / SIFT_HIP_CAP_DIVISOR override them (read on every
+// call) so that tests can force the overflow-and-retry path.
+static size_t initial_cap_min() {
+    const char* e = getenv("SIFT_HIP_CAP_MIN");
+    const long v = e ? atol(e) : 0;
+    return v > 0 ? (size_t)v : 4096;
+}
+static size_t initial_cap_divisor() {
+    const char* e = getenv("SIFT_HIP_CAP_DIVISOR");
+    const long v = e ? atol(e) : 0;
+    return v > 0 ? (size_t)v : 128;
+}
+
 void Context::clear_graphs() {
     for (GraphEntry& e : graphs) (void)hipGraphExecDestroy(e.exec);
     graphs.clear();
@@ -495,7 +509,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     const int noct = (int)octs.size();
 
     // capacities (grow-and-retry on overflow)
-    size_t want_cap = std::max<size_t>(4096, pix_layers / 128);
+    size_t want_cap = std::max<size_t>(initial_cap_min(), pix_layers / initial_cap_divisor());
     if (cand_cap < want_cap) cand_cap = want_cap;
     if (kp_cap < cand_cap + cand_cap / 2) kp_cap = cand_cap + cand_cap / 2;
     const int cap = (int)cand_cap, kcap = (int)kp_cap;
