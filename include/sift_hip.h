@@ -15,6 +15,11 @@
  *                           (with the initial blur)        src/sift.rs:129-204,1205-1214
  *   sift_hip_detect_stages <- find_scale_space_extrema / refine_and_filter_extrema
  *                           stage outputs                  src/sift.rs:510-603,299-474
+ *   sift_hip_to_luma8,    <- DynamicImage::to_luma8 before the CPU path src/sift.rs:1202,
+ *   sift_hip_detect*_color   img_from_raw of the wasm entry          src/wasm.rs:79-103
+ *   sift_hip_detect_batch_u8 <- the WebGPU arms' u8 descriptors and V2 keypoints
+ *                           src/shaders/descriptor.wgsl:256-262, src/sift.rs:1288-1291,
+ *                           src/gpu_sift_v2.rs:1699-1707
  */
 #ifndef SIFT_HIP_H
 #define SIFT_HIP_H
@@ -175,6 +180,52 @@ typedef struct sift_completion_t {
     double submit_ms, start_ms, done_ms;  /* steady-clock ms, for latency accounting */
 } sift_completion_t;
 
+/* ---- Colour ingest and the WebGPU backends' output conventions ----------
+ * The CPU path starts with DynamicImage::to_luma8 (src/sift.rs:1202); these
+ * entry points take the colour pixels and convert them on the device. */
+#define SIFT_PIXEL_GRAY8 0        /* one byte per pixel, used as is */
+#define SIFT_PIXEL_RGB8 1         /* R,G,B: to_luma8 of image 0.25.6, (2126 R + 7152 G + 722 B) / 10000 */
+#define SIFT_PIXEL_RGBA8 2        /* R,G,B,A: the same, alpha ignored */
+#define SIFT_PIXEL_LUMAA8 3       /* L,A: L */
+#define SIFT_PIXEL_RGBA8_CANVAS 4 /* R,G,B,A: the wasm entry's conversion (src/wasm.rs:79-94),
+                                     (0.299 r + 0.587 g + 0.114 b) in f32, truncated to u8 */
+
+/* Row-major pixels of `format`; stride = bytes between rows (0 means
+ * width * bytes per pixel). */
+typedef struct sift_color_image_t {
+    const uint8_t* data;
+    uint32_t width, height, stride;
+    uint32_t format;
+} sift_color_image_t;
+
+/* Grey image of one colour image (width*height bytes, no padding), converted
+ * on the device: to_luma8 / img_from_raw (src/sift.rs:1202, src/wasm.rs:79-94). */
+int sift_hip_to_luma8(sift_hip_ctx* ctx, const sift_color_image_t* image, uint8_t* out);
+
+/* sift_hip_detect / _batch on colour images: the result equals that of the grey
+ * image sift_hip_to_luma8 gives. */
+int sift_hip_detect_color(sift_hip_ctx* ctx, const sift_color_image_t* image, sift_result_t* out);
+int sift_hip_detect_batch_color(sift_hip_ctx* ctx, const sift_color_image_t* images,
+                                const sift_params_t* params, size_t n, sift_result_t* out);
+
+/* Descriptors in the u8 form of the WebGPU backends (src/shaders/descriptor.wgsl:256-262):
+ * u8 = trunc(clamp(d * 512, 0, 255)) of the CPU-convention descriptor d; the
+ * reference hands them to its callers as u8 / 255 (src/sift.rs:1288-1291).
+ * 128 B per keypoint instead of 512 B, also on the device-to-host copy. */
+#define SIFT_KEYPOINTS_CPU 0      /* keypoints as the CPU path gives them */
+#define SIFT_KEYPOINTS_WGPU_V2 1  /* as the V2 backend reports them (src/gpu_sift_v2.rs:1699-1707):
+                                     size = 2 * sigma; response, octave and layer zero */
+typedef struct sift_result_u8_t {
+    sift_keypoint_t* keypoints;
+    uint8_t* descriptors;  /* count x 128 */
+    size_t count;
+} sift_result_u8_t;
+
+int sift_hip_detect_batch_u8(sift_hip_ctx* ctx, const sift_color_image_t* images,
+                             const sift_params_t* params, size_t n, uint32_t keypoint_convention,
+                             sift_result_u8_t* out);
+void sift_result_u8_free(sift_result_u8_t* r);
+
 int sift_hip_server_start(sift_hip_ctx* ctx, const sift_server_config_t* cfg);
 /* Copies the pixels before returning. params: NULL = the context's params,
  * else this frame's (e.g. an octave count chosen by frame size).
@@ -182,6 +233,9 @@ int sift_hip_server_start(sift_hip_ctx* ctx, const sift_server_config_t* cfg);
  * exceeds the HBM budget. */
 int sift_hip_submit(sift_hip_ctx* ctx, const sift_image_t* image, const sift_params_t* params,
                     uint64_t tag);
+/* sift_hip_submit for a colour frame (converted to grey on the device). */
+int sift_hip_submit_color(sift_hip_ctx* ctx, const sift_color_image_t* image,
+                          const sift_params_t* params, uint64_t tag);
 int sift_hip_poll(sift_hip_ctx* ctx, sift_completion_t* out, size_t cap, size_t* n,
                   int timeout_ms);
 int sift_hip_server_stop(sift_hip_ctx* ctx);

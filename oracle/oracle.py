@@ -70,6 +70,9 @@ def lib():
         L.oracle_orientation_single.restype = ctypes.c_int
         L.oracle_descriptor_single.argtypes = [P(OracleParams), u8p, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_void_p, f32p]
+        L.oracle_to_luma8.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
+                                      u8p]
+        L.oracle_descriptor_u8.argtypes = [f32p, ctypes.c_size_t, u8p]
         _LIB = L
     return _LIB
 
@@ -202,6 +205,31 @@ def detect_and_compute(p, img, nthreads=0, stages=False):
     if stages:
         return kps, desc, c, r, list(ms)
     return kps, desc
+
+
+# pixel formats (the SIFT_PIXEL_* values of the C ABI)
+GRAY8, RGB8, RGBA8, LUMAA8, RGBA8_CANVAS = 0, 1, 2, 3, 4
+_BPP = {GRAY8: 1, RGB8: 3, RGBA8: 4, LUMAA8: 2, RGBA8_CANVAS: 4}
+
+
+def to_luma8(img, fmt):
+    """Grey image of an (H, W, C) u8 array in pixel format `fmt`: to_luma8 of
+    image 0.25.6 (src/sift.rs:1202) or the wasm canvas rule (src/wasm.rs:79-94)."""
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape[:2]
+    if img.size != h * w * _BPP[fmt]:
+        raise ValueError("array shape does not match the pixel format")
+    out = np.empty((h, w), np.uint8)
+    lib().oracle_to_luma8(_u8(img), w, h, w * _BPP[fmt], fmt, _u8(out))
+    return out
+
+
+def descriptor_u8(desc):
+    """u8 descriptors of the WebGPU backends (src/shaders/descriptor.wgsl:256-262)."""
+    d = np.ascontiguousarray(desc, np.float32)
+    out = np.empty(d.shape, np.uint8)
+    lib().oracle_descriptor_u8(_f32(d), d.size, _u8(out))
+    return out
 
 
 def solve_linear_system(a, b):

@@ -26,6 +26,16 @@
  *  - image resize_exact Lanczos3 (crate 0.25.6, source not available here):
  *    flat fields, and PIL's Lanczos resample (same window/normalisation rule,
  *    fixed-point arithmetic) within 1 grey level. Exact rounding is UNPINNED.
+ *  - to_luma8 (image 0.25.6 color.rs rgb_to_luma, source not available here):
+ *    grey inputs map to themselves (the weights sum to the divisor), the
+ *    primaries give the Rec. 709 luma of 255 truncated (54, 182, 18), and every
+ *    one of the 2^24 RGB triples lies within [exact - 1, exact] of the real
+ *    Rec. 709 value. The integer weights themselves are the crate's published
+ *    constants; that the crate truncates (not rounds) is UNPINNED by any
+ *    reference test.
+ *  - the wasm canvas conversion (src/wasm.rs:88-94, in the reference itself):
+ *    all 2^24 triples against exact rational arithmetic on the f32 constants,
+ *    which fixes every result except where the f32 sum rounds across an integer.
  */
 #define _GNU_SOURCE
 #include "sift_oracle.h"
@@ -877,3 +887,43 @@ void oracle_descriptor_single(const oracle_params_t* p, const uint8_t* g, int w,
 }
 
 void oracle_free(void* ptr) { free(ptr); }
+
+/* --------------------------------------------------------------------------
+ * Colour ingest and the WebGPU output convention
+ * ------------------------------------------------------------------------ */
+
+/* image 0.25.6 color.rs rgb_to_luma for u8: the u32 sum
+ * 2126 R + 7152 G + 722 B divided by 10000 (integer division, truncating);
+ * Rgba and LumaA drop alpha. Canvas RGBA: src/wasm.rs:88-94,
+ * ((0.299 r + 0.587 g) + 0.114 b) in f32 (Rust evaluates left to right, no
+ * FMA), then `luma as u8`. */
+void oracle_to_luma8(const uint8_t* src, int w, int h, size_t stride, int format, uint8_t* dst) {
+    const int bpp = format == 1 ? 3 : format == 3 ? 2 : format == 0 ? 1 : 4;
+    for (int y = 0; y < h; y++) {
+        const uint8_t* row = src + (size_t)y * stride;
+        for (int x = 0; x < w; x++) {
+            const uint8_t* p = row + (size_t)x * bpp;
+            uint8_t l;
+            if (format == 0 || format == 3) {
+                l = p[0];
+            } else if (format == 4) {
+                const float r = (float)p[0], g = (float)p[1], b = (float)p[2];
+                const float v = 0.299f * r + 0.587f * g + 0.114f * b;
+                l = clamp_u8_trunc(v);
+            } else {
+                l = (uint8_t)((2126u * p[0] + 7152u * p[1] + 722u * p[2]) / 10000u);
+            }
+            dst[(size_t)y * w + x] = l;
+        }
+    }
+}
+
+/* descriptor.wgsl:258-261: u32(clamp(normalized * 512.0, 0.0, 255.0)); the
+ * conversion to u32 truncates. */
+void oracle_descriptor_u8(const float* desc, size_t n, uint8_t* out) {
+    for (size_t i = 0; i < n; i++) {
+        float v = desc[i] * 512.0f;
+        v = v < 0.0f ? 0.0f : (v > 255.0f ? 255.0f : v);
+        out[i] = (uint8_t)v;
+    }
+}

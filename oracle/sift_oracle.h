@@ -91,6 +91,16 @@ int oracle_detect_and_compute(const oracle_params_t* p, const uint8_t* img, int 
 
 void oracle_free(void* ptr);
 
+/* Grey image of a colour input (w*h bytes out, no padding). format: 0 grey,
+ * 1 RGB8, 2 RGBA8, 3 LumaA8 -- DynamicImage::to_luma8 of image 0.25.6, called
+ * at src/sift.rs:1202 -- and 4, the wasm entry's RGBA canvas conversion
+ * (src/wasm.rs:79-94). stride = bytes between input rows. */
+void oracle_to_luma8(const uint8_t* src, int w, int h, size_t stride, int format, uint8_t* dst);
+
+/* u8 descriptors of the WebGPU backends (src/shaders/descriptor.wgsl:256-262):
+ * u32(clamp(d * 512, 0, 255)) per value, n values. */
+void oracle_descriptor_u8(const float* desc, size_t n, uint8_t* out);
+
 /* Unit-level entry points (for the reference's own known-answer tests). */
 int oracle_solve_linear_system(const float a[9], const float b[3], float x[3]);
 float oracle_gauss_pixel_bilinear(const uint8_t* img, int w, int h, float x, float y);

@@ -344,14 +344,15 @@ struct LaunchRec {
 enum KernelId {
     K_BLUR_H = 0, K_BLUR_V, K_COPY, K_BLUR_H_GENERIC, K_RESIZE_V, K_RESIZE_H, K_EXTREMA, K_SCAN,
     K_COMPACT, K_REFINE, K_ORIENT, K_DESCRIPTOR, K_FINALIZE, K_BLUR_V_NARROW, K_BLUR_V_GENERIC,
-    K_BLUR_H2, K_BLUR_H_SMALL, K_BLUR_V_SMALL, K_COUNT
+    K_BLUR_H2, K_BLUR_H_SMALL, K_BLUR_V_SMALL, K_LUMA, K_DESC_U8, K_COUNT
 };
 // Names as rocprofv3 reports the kernels (template arguments included).
 const char* const kKernelNames[K_COUNT] = {
     "k_blur_h<8>", "k_blur_v", "k_copy_level", "k_blur_h_generic", "k_resize_v", "k_resize_h",
     "k_extrema_tile", "k_scan", "k_compact", "k_refine", "k_orient", "k_descriptor",
     "k_finalize", "k_blur_v_narrow", "k_blur_v_generic", "k_blur_h<2>", "k_blur_h_small",
-    "k_blur_v_small"};
+    "k_blur_v_small", "k_luma", "k_desc_u8"};
+
 
 double blur_job_bytes(const BlurJob& j, int kind) {
     double a = (double)j.w * j.h;
@@ -360,6 +361,9 @@ double blur_job_bytes(const BlurJob& j, int kind) {
         return a * (2.0 + 1.0 + 4.0);
     if (kind == K_COPY && j.dog) return a * (2.0 + 4.0);
     return a * 2.0;
+}
+double luma_job_bytes(const LumaJob& j) {
+    return (double)j.w * j.h * (pixel_bytes((uint32_t)j.format) + 1);
 }
 double resize_job_bytes(const ResizeJob& j, int kind) {
     if (kind == K_RESIZE_V) return (double)j.in_w * j.in_h + (double)j.in_w * j.out_h * 4.0;
@@ -448,14 +452,28 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     std::vector<size_t> in_off(n), htemp_off(n), rtmp_off(n);
     std::vector<std::vector<size_t>> lev_off(n), dog_off(n);
     std::vector<int> in_pitch(n);
+    // colour inputs: raw pixels (host inputs are staged at color_off) and the
+    // grey image k_luma writes at in_off
+    std::vector<size_t> color_off(n, 0);
+    std::vector<int> color_pitch(n, 0);
     for (int i = 0; i < n; i++) {
         const sift_image_t& im = B.images[i];
+        const uint32_t fmt = B.format(i);
         int S = (int)B.params[i].num_intervals, L = S + 3;
-        if (B.device_input) {
+        if (B.device_input && fmt == SIFT_PIXEL_GRAY8) {
             in_pitch[i] = (int)(im.stride ? im.stride : im.width);
         } else {
             in_pitch[i] = (int)round_up(im.width, 64);
             in_off[i] = C.take((size_t)in_pitch[i] * im.height);
+        }
+        if (fmt != SIFT_PIXEL_GRAY8) {
+            const int bpp = pixel_bytes(fmt);
+            if (B.device_input) {
+                color_pitch[i] = (int)(im.stride ? im.stride : (size_t)im.width * bpp);
+            } else {
+                color_pitch[i] = (int)round_up((size_t)im.width * bpp, 64);
+                color_off[i] = C.take((size_t)color_pitch[i] * im.height);
+            }
         }
         lev_off[i].resize(shapes[i].size());
         dog_off[i].resize(shapes[i].size());
@@ -530,6 +548,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     const size_t desc_off = C.take((size_t)kcap * 128 * 4);
     const size_t summary_off = C.take((size_t)(n + 3) * 4);
     const size_t wide_off = C.take((size_t)cap * 4);  // keypoints for the wide-window descriptor
+    const size_t descu8_off = B.desc_u8 ? C.take((size_t)kcap * 128) : 0;
 
     // --- job blob (host), uploaded to the end of the arena ---
     std::vector<char> blob;
@@ -562,6 +581,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     auto job_bytes = [](const auto& j, int kind) -> double {
         using J = std::decay_t<decltype(j)>;
         if constexpr (std::is_same_v<J, BlurJob>) return blur_job_bytes(j, kind);
+        else if constexpr (std::is_same_v<J, LumaJob>) return luma_job_bytes(j);
         else return resize_job_bytes(j, kind);
     };
     auto push_launch = [&](int kind, auto& jobs, int max_pad, int variant) {
@@ -674,10 +694,34 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         vn_rows = hs_taps4 = vs_taps4 = vs_rows = 0;
     };
 
+    // colour -> grey (to_luma8 / the canvas rule)
+    {
+        std::vector<LumaJob> lj;
+        for (int i = 0; i < n; i++) {
+            const uint32_t fmt = B.format(i);
+            if (fmt == SIFT_PIXEL_GRAY8) continue;
+            const sift_image_t& im = B.images[i];
+            LumaJob J{};
+            J.src = B.device_input ? im.data : (const uint8_t*)AP(color_off[i]);
+            J.src_pitch = color_pitch[i];
+            J.dst = (uint8_t*)AP(in_off[i]);
+            J.dst_pitch = in_pitch[i];
+            J.w = (int)im.width;
+            J.h = (int)im.height;
+            J.format = (int)fmt;
+            J.tiles_x = (J.w + kLumaTileW - 1) / kLumaTileW;
+            J.tiles = J.tiles_x * J.h;
+            lj.push_back(J);
+        }
+        push_launch(K_LUMA, lj, 0, 0);
+    }
+
     // initial blur into octave 0 level 0
     for (int i = 0; i < n; i++) {
         const OctShape& s = shapes[i][0];
-        const uint8_t* src = B.device_input ? B.images[i].data : (const uint8_t*)AP(in_off[i]);
+        const uint8_t* src = B.device_input && B.format(i) == SIFT_PIXEL_GRAY8
+                                 ? B.images[i].data
+                                 : (const uint8_t*)AP(in_off[i]);
         make_blur_pair(src, in_pitch[i], (uint8_t*)AP(htemp_off[i]), s.pitch,
                        (uint8_t*)AP(lev_off[i][0]), s.pitch, s.w, s.h, ptab[i]->initial, nullptr,
                        nullptr, 0);
@@ -761,7 +805,8 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     // staging: inputs + blob in pinned memory
     size_t in_bytes = 0;
     if (!B.device_input)
-        for (int i = 0; i < n; i++) in_bytes += (size_t)B.images[i].width * B.images[i].height;
+        for (int i = 0; i < n; i++)
+            in_bytes += (size_t)B.images[i].width * B.images[i].height * pixel_bytes(B.format(i));
     rc = ensure_pinned(round_up(blob.size(), 256) + in_bytes + 256);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(stream));  // pinned staging is reused across calls
@@ -772,12 +817,18 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         size_t po = round_up(blob.size(), 256);
         for (int i = 0; i < n; i++) {
             const sift_image_t& im = B.images[i];
-            const size_t stride = im.stride ? im.stride : im.width;
+            const uint32_t fmt = B.format(i);
+            const size_t row = (size_t)im.width * pixel_bytes(fmt);
+            const size_t stride = im.stride ? im.stride : row;
             for (uint32_t y = 0; y < im.height; y++)
-                memcpy(pinned + po + (size_t)y * im.width, im.data + (size_t)y * stride, im.width);
-            HIP_TRY(hipMemcpy2DAsync(A + in_off[i], in_pitch[i], pinned + po, im.width, im.width,
-                                     im.height, hipMemcpyHostToDevice, st));
-            po += (size_t)im.width * im.height;
+                memcpy(pinned + po + (size_t)y * row, im.data + (size_t)y * stride, row);
+            if (fmt == SIFT_PIXEL_GRAY8)
+                HIP_TRY(hipMemcpy2DAsync(A + in_off[i], in_pitch[i], pinned + po, row, row, im.height,
+                                         hipMemcpyHostToDevice, st));
+            else
+                HIP_TRY(hipMemcpy2DAsync(A + color_off[i], color_pitch[i], pinned + po, row, row,
+                                         im.height, hipMemcpyHostToDevice, st));
+            po += row * im.height;
         }
     }
     char* D = A + blob_dev_off;
@@ -827,6 +878,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
                     launch_blur_v_small((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.max_pad,
                                         L.variant, st);
                     break;
+                case K_LUMA: launch_luma((const LumaJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
                 case K_RESIZE_V: launch_resize_v((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
                 case K_RESIZE_H: launch_resize_h((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
             }
@@ -871,6 +923,11 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         prof_begin(K_FINALIZE, 0.0, &ev);
         launch_finalize(d_offs, (const int*)(D + isb_blob), n, d_koff, d_total, cap, d_sum, st);
         prof_end(K_FINALIZE, 0.0, ev);
+        if (B.desc_u8) {
+            prof_begin(K_DESC_U8, 0.0, &ev);
+            launch_desc_u8(d_desc, d_sum + 1, kcap, (uint8_t*)AP(descu8_off), st);
+            prof_end(K_DESC_U8, 0.0, ev);
+        }
         HIP_TRY(hipGetLastError());
         return SIFT_OK;
     };
@@ -887,7 +944,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
                                      oct_blob, par_blob, isb_blob, bits_off, counts_off, offs_off,
                                      scantmp1_off, cand_off, candresp_off, ref_off, valid_off, npk_off,
                                      ang_off, koff_off, scantmp2_off, kps_off, desc_off, summary_off,
-                                     wide_off};
+                                     wide_off, (uint64_t)B.desc_u8, descu8_off};
         for (const LaunchRec& L : launches) {
             key.push_back((uint64_t)L.kind << 32 | (uint32_t)L.njobs);
             key.push_back(L.job_off);
@@ -950,6 +1007,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     if (outs) {
         outs->d_kps = d_kps;
         outs->d_desc = d_desc;
+        outs->d_desc_u8 = B.desc_u8 ? (uint8_t*)AP(descu8_off) : nullptr;
         outs->d_summary = d_sum;
         outs->d_cand = d_cand;
         outs->d_cand_resp = (float*)AP(candresp_off);
@@ -989,7 +1047,21 @@ int check_image(const sift_image_t* im) {
     return SIFT_OK;
 }
 
-int collect(sh::Context* C, int n, sh::RunOutputs& R, sift_result_t* out, bool* retry) {
+int check_color_image(const sift_color_image_t* im) {
+    if (!im || !im->data) return fail(SIFT_ERR_INVALID, "image or image data is null");
+    const int bpp = sh::pixel_bytes(im->format);
+    if (bpp == 0) return fail(SIFT_ERR_INVALID, "unknown pixel format");
+    if (im->width == 0 || im->height == 0) return fail(SIFT_ERR_INVALID, "image has zero size");
+    if (im->width > 65536 || im->height > 65536) return fail(SIFT_ERR_INVALID, "image too large");
+    if (im->stride && im->stride < (size_t)im->width * bpp)
+        return fail(SIFT_ERR_INVALID, "stride < width * bytes per pixel");
+    return SIFT_OK;
+}
+
+// Copies the results out: f32 descriptors into `out`, or (when out8 is set)
+// the u8 descriptors into `out8`, with the keypoint convention kp_conv.
+int collect(sh::Context* C, int n, sh::RunOutputs& R, sift_result_t* out, bool* retry,
+            sift_result_u8_t* out8 = nullptr, uint32_t kp_conv = SIFT_KEYPOINTS_CPU) {
     std::vector<unsigned> sum(n + 3);
     HIP_TRY(hipMemcpyAsync(sum.data(), R.d_summary, (n + 3) * 4, hipMemcpyDeviceToHost, C->stream));
     HIP_TRY(hipStreamSynchronize(C->stream));
@@ -1001,16 +1073,39 @@ int collect(sh::Context* C, int n, sh::RunOutputs& R, sift_result_t* out, bool* 
         *retry = true;
         return SIFT_OK;
     }
-    if (!out) return SIFT_OK;
-    const size_t kbytes = nkp * sizeof(sh::Keypoint), dbytes = nkp * 128 * 4;
+    if (!out && !out8) return SIFT_OK;
+    const size_t dsz = out8 ? 128 : 128 * 4;
+    const size_t kbytes = nkp * sizeof(sh::Keypoint), dbytes = nkp * dsz;
     int rc = C->ensure_pinned_out(kbytes + dbytes + 256);
     if (rc) return rc;
-    const sh::Keypoint* kps = (const sh::Keypoint*)C->pinned_out;
-    const float* desc = (const float*)(C->pinned_out + ((kbytes + 255) / 256) * 256);
+    sh::Keypoint* kps = (sh::Keypoint*)C->pinned_out;
+    char* desc = C->pinned_out + ((kbytes + 255) / 256) * 256;
     if (nkp) {
         HIP_TRY(hipMemcpyAsync((void*)kps, R.d_kps, kbytes, hipMemcpyDeviceToHost, C->stream));
-        HIP_TRY(hipMemcpyAsync((void*)desc, R.d_desc, dbytes, hipMemcpyDeviceToHost, C->stream));
+        HIP_TRY(hipMemcpyAsync((void*)desc, out8 ? (const void*)R.d_desc_u8 : (const void*)R.d_desc,
+                               dbytes, hipMemcpyDeviceToHost, C->stream));
         HIP_TRY(hipStreamSynchronize(C->stream));
+    }
+    if (kp_conv == SIFT_KEYPOINTS_WGPU_V2)
+        for (size_t k = 0; k < nkp; k++) {
+            kps[k].size = kps[k].size * 2.0f;
+            kps[k].response = 0.0f;
+            kps[k].octave = 0;
+            kps[k].layer = 0;
+        }
+    if (out8) {
+        for (int i = 0; i < n; i++) {
+            size_t b = sum[2 + i], c = sum[3 + i] - b;
+            out8[i] = sift_result_u8_t{nullptr, nullptr, c};
+            if (c) {
+                out8[i].keypoints = (sift_keypoint_t*)malloc(c * sizeof(sift_keypoint_t));
+                out8[i].descriptors = (uint8_t*)malloc(c * 128);
+                if (!out8[i].keypoints || !out8[i].descriptors) return fail(SIFT_ERR_NOMEM, "host allocation failed");
+                memcpy(out8[i].keypoints, kps + b, c * sizeof(sift_keypoint_t));
+                memcpy(out8[i].descriptors, desc + b * 128, c * 128);
+            }
+        }
+        return SIFT_OK;
     }
     for (int i = 0; i < n; i++) {
         size_t b = sum[2 + i], e = sum[3 + i];
@@ -1023,7 +1118,7 @@ int collect(sh::Context* C, int n, sh::RunOutputs& R, sift_result_t* out, bool* 
             out[i].descriptors = (float*)malloc(c * 128 * sizeof(float));
             if (!out[i].keypoints || !out[i].descriptors) return fail(SIFT_ERR_NOMEM, "host allocation failed");
             memcpy(out[i].keypoints, kps + b, c * sizeof(sift_keypoint_t));
-            memcpy(out[i].descriptors, desc + b * 128, c * 128 * sizeof(float));
+            memcpy(out[i].descriptors, desc + b * 128 * 4, c * 128 * sizeof(float));
         }
     }
     return SIFT_OK;
@@ -1034,15 +1129,19 @@ int collect(sh::Context* C, int n, sh::RunOutputs& R, sift_result_t* out, bool* 
 namespace sh {
 // Runs a host-input batch and copies the results out, growing the candidate
 // and keypoint capacities and re-running when a count overflowed them.
-int run_and_collect(Context* C, const Batch& B, sift_result_t* out) {
+int run_and_collect(Context* C, const Batch& B, sift_result_t* out, sift_result_u8_t* out8,
+                    uint32_t kp_conv) {
     const int n = (int)B.images.size();
-    for (int i = 0; i < n; i++) out[i] = sift_result_t{nullptr, nullptr, 0};
+    for (int i = 0; i < n; i++) {
+        if (out) out[i] = sift_result_t{nullptr, nullptr, 0};
+        if (out8) out8[i] = sift_result_u8_t{nullptr, nullptr, 0};
+    }
     for (int attempt = 0; attempt < 4; attempt++) {
         RunOutputs R;
         int rc = C->run(B, &R);
         if (rc) return rc;
         bool retry = false;
-        rc = collect(C, n, R, out, &retry);
+        rc = collect(C, n, R, out, &retry, out8, kp_conv);
         if (rc) return rc;
         if (!retry) return SIFT_OK;
     }
@@ -1123,6 +1222,113 @@ int sift_hip_detect_batch(sift_hip_ctx* ctx, const sift_image_t* images, const s
 
 int sift_hip_detect(sift_hip_ctx* ctx, const sift_image_t* image, sift_result_t* out) {
     return sift_hip_detect_batch(ctx, image, nullptr, 1, out);
+}
+
+namespace {
+// Checks colour images and params and fills the batch (images, formats, params).
+int color_batch(sh::Context* C, const sift_color_image_t* images, const sift_params_t* params,
+                size_t n, sh::Batch* B) {
+    B->images.resize(n);
+    B->formats.resize(n);
+    B->params.assign(n, C->params);
+    for (size_t i = 0; i < n; i++) {
+        int rc = check_color_image(&images[i]);
+        if (rc) return rc;
+        if (params && (rc = check_params(&params[i]))) return rc;
+        const sift_color_image_t& im = images[i];
+        B->images[i] = sift_image_t{im.data, im.width, im.height, im.stride};
+        B->formats[i] = im.format;
+        if (params) B->params[i] = params[i];
+    }
+    return SIFT_OK;
+}
+}  // namespace
+
+int sift_hip_detect_batch_color(sift_hip_ctx* ctx, const sift_color_image_t* images,
+                                const sift_params_t* params, size_t n, sift_result_t* out) {
+    if (!ctx || !images || !out || n == 0) return fail(SIFT_ERR_INVALID, "null argument or empty batch");
+    for (size_t i = 0; i < n; i++) out[i] = sift_result_t{nullptr, nullptr, 0};
+    sh::Context* C = reinterpret_cast<sh::Context*>(ctx);
+    sh::Batch B;
+    int rc = color_batch(C, images, params, n, &B);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(C->mu);
+    (void)hipSetDevice(C->device);
+    return sh::run_and_collect(C, B, out);
+}
+
+int sift_hip_detect_color(sift_hip_ctx* ctx, const sift_color_image_t* image, sift_result_t* out) {
+    return sift_hip_detect_batch_color(ctx, image, nullptr, 1, out);
+}
+
+int sift_hip_detect_batch_u8(sift_hip_ctx* ctx, const sift_color_image_t* images,
+                             const sift_params_t* params, size_t n, uint32_t keypoint_convention,
+                             sift_result_u8_t* out) {
+    if (!ctx || !images || !out || n == 0) return fail(SIFT_ERR_INVALID, "null argument or empty batch");
+    for (size_t i = 0; i < n; i++) out[i] = sift_result_u8_t{nullptr, nullptr, 0};
+    if (keypoint_convention != SIFT_KEYPOINTS_CPU && keypoint_convention != SIFT_KEYPOINTS_WGPU_V2)
+        return fail(SIFT_ERR_INVALID, "unknown keypoint convention");
+    sh::Context* C = reinterpret_cast<sh::Context*>(ctx);
+    sh::Batch B;
+    int rc = color_batch(C, images, params, n, &B);
+    if (rc) return rc;
+    B.desc_u8 = true;
+    std::lock_guard<std::mutex> lk(C->mu);
+    (void)hipSetDevice(C->device);
+    return sh::run_and_collect(C, B, nullptr, out, keypoint_convention);
+}
+
+void sift_result_u8_free(sift_result_u8_t* r) {
+    if (!r) return;
+    free(r->keypoints);
+    free(r->descriptors);
+    r->keypoints = nullptr;
+    r->descriptors = nullptr;
+    r->count = 0;
+}
+
+int sift_hip_to_luma8(sift_hip_ctx* ctx, const sift_color_image_t* image, uint8_t* out) {
+    if (!ctx || !out) return fail(SIFT_ERR_INVALID, "null argument");
+    int rc = check_color_image(image);
+    if (rc) return rc;
+    sh::Context* C = reinterpret_cast<sh::Context*>(ctx);
+    std::lock_guard<std::mutex> lk(C->mu);
+    (void)hipSetDevice(C->device);
+    const uint32_t w = image->width, h = image->height;
+    const size_t row = (size_t)w * sh::pixel_bytes(image->format);
+    const size_t stride = image->stride ? image->stride : row;
+    const size_t cpitch = round_up(row, 64), gpitch = round_up(w, 64);
+    sh::Carver L;
+    const size_t job_off = L.take(sizeof(sh::LumaJob));
+    const size_t color_off = L.take(cpitch * h);
+    const size_t gray_off = L.take(gpitch * h);
+    if ((rc = C->ensure_arena(L.off))) return rc;
+    if ((rc = C->ensure_pinned(256 + row * h))) return rc;
+    if ((rc = C->ensure_pinned_out((size_t)w * h))) return rc;
+    HIP_TRY(hipStreamSynchronize(C->stream));  // pinned staging is reused across calls
+    sh::LumaJob J{};
+    J.src = (const uint8_t*)C->arena + color_off;
+    J.dst = (uint8_t*)C->arena + gray_off;
+    J.w = (int)w;
+    J.h = (int)h;
+    J.src_pitch = (int)cpitch;
+    J.dst_pitch = (int)gpitch;
+    J.format = (int)image->format;
+    J.tiles_x = (J.w + sh::kLumaTileW - 1) / sh::kLumaTileW;
+    J.tiles = J.tiles_x * J.h;
+    J.tile_begin = 0;
+    memcpy(C->pinned, &J, sizeof(J));
+    for (uint32_t y = 0; y < h; y++) memcpy(C->pinned + 256 + y * row, image->data + y * stride, row);
+    HIP_TRY(hipMemcpyAsync(C->arena + job_off, C->pinned, sizeof(J), hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemcpy2DAsync(C->arena + color_off, cpitch, C->pinned + 256, row, row, h,
+                             hipMemcpyHostToDevice, C->stream));
+    sh::launch_luma((const sh::LumaJob*)(C->arena + job_off), 1, J.tiles, C->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy2DAsync(C->pinned_out, w, C->arena + gray_off, gpitch, w, h, hipMemcpyDeviceToHost,
+                             C->stream));
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    memcpy(out, C->pinned_out, (size_t)w * h);
+    return SIFT_OK;
 }
 
 int sift_hip_detect_batch_device(sift_hip_ctx* ctx, const sift_image_t* images,

@@ -11,6 +11,8 @@
 // Gaussian levels as q[G_{s+1}] - q[G_s] with q[v] = v / 255 (IEEE division,
 // kept in an LDS table), which is bit-identical to the reference's
 // (G_{s+1} as f32 / 255.0) - (G_s as f32 / 255.0) (src/sift.rs:97,120).
+#include <algorithm>
+
 #include "sift_internal.h"
 
 namespace sh {
@@ -1258,6 +1260,24 @@ __global__ void k_finalize(const unsigned* __restrict__ slot_offsets,
     }
 }
 
+// u8 descriptors of the WebGPU backends (src/shaders/descriptor.wgsl:256-262):
+// trunc(clamp(d * 512, 0, 255)) of each value, 4 values per lane.
+__device__ __forceinline__ unsigned desc_u8_of(float d) {
+    return (unsigned)fminf(fmaxf(d * 512.0f, 0.0f), 255.0f);
+}
+
+__global__ __launch_bounds__(256) void k_desc_u8(const float4* __restrict__ desc,
+                                                 const unsigned* __restrict__ count, int kcap,
+                                                 unsigned* __restrict__ out) {
+    const size_t words = (size_t)min(*count, (unsigned)kcap) * 32;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const float4 v = desc[i];
+        out[i] = desc_u8_of(v.x) | (desc_u8_of(v.y) << 8) | (desc_u8_of(v.z) << 16) |
+                 (desc_u8_of(v.w) << 24);
+    }
+}
+
 }  // namespace
 
 void launch_extrema(const OctaveDesc* d_oct, int noct, int total_rows, const ImageParams* d_params,
@@ -1344,6 +1364,14 @@ void launch_finalize(const unsigned* d_slot_offsets, const int* d_img_slot_begin
                      hipStream_t s) {
     hipLaunchKernelGGL(k_finalize, dim3((nimg + 1 + 255) / 256), dim3(256), 0, s, d_slot_offsets,
                        d_img_slot_begin, nimg, d_koff, d_total, cap, d_summary);
+}
+
+void launch_desc_u8(const float* d_desc, const unsigned* d_count, int kcap, uint8_t* d_out,
+                    hipStream_t s) {
+    if (kcap <= 0) return;
+    const int blocks = (int)std::min<long long>(1024, ((long long)kcap * 32 + 255) / 256);
+    hipLaunchKernelGGL(k_desc_u8, dim3(blocks), dim3(256), 0, s, (const float4*)d_desc, d_count, kcap,
+                       (unsigned*)d_out);
 }
 
 }  // namespace sh

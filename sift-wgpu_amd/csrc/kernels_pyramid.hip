@@ -8,6 +8,7 @@
 // in tap order with separate multiply and add (no FMA; the file is compiled with
 // -ffp-contract=off and uses __fmul_rn/__fadd_rn), then truncates to u8 -- the
 // reference's arithmetic, so the levels are bit-identical to it.
+#include "sift_hip.h"
 #include "sift_internal.h"
 
 namespace sh {
@@ -466,6 +467,55 @@ __global__ __launch_bounds__(kThreads) void k_copy_level(const BlurJob* __restri
 }
 
 // ---------------------------------------------------------------------------
+// Colour -> grey before the pyramid. RGB8 / RGBA8: DynamicImage::to_luma8 of
+// image 0.25.6 (color.rs rgb_to_luma: u32 sum 2126 R + 7152 G + 722 B, divided
+// by 10000 with truncation; alpha dropped), the conversion at src/sift.rs:1202.
+// LUMAA8: the luma byte. RGBA8_CANVAS: src/wasm.rs:88-94, f32
+// (0.299 r + 0.587 g) + 0.114 b without FMA, then `as u8` (truncate, saturate).
+// A lane converts 4 consecutive pixels of one row.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned luma_srgb(unsigned r, unsigned g, unsigned b) {
+    return (2126u * r + 7152u * g + 722u * b) / 10000u;
+}
+
+__device__ __forceinline__ unsigned luma_canvas(unsigned r, unsigned g, unsigned b) {
+    const float l = __fadd_rn(__fadd_rn(__fmul_rn(0.299f, (float)r), __fmul_rn(0.587f, (float)g)),
+                              __fmul_rn(0.114f, (float)b));
+    return u8_trunc(l);
+}
+
+__global__ __launch_bounds__(kThreads) void k_luma(const LumaJob* __restrict__ jobs, int njobs) {
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const LumaJob& J = jobs[find_job(jobs, njobs, tile)];
+    const int t = tile - J.tile_begin;
+    const int y = t / J.tiles_x;
+    const int x = (t % J.tiles_x) * kLumaTileW + threadIdx.x * 4;
+    if (x >= J.w || y >= J.h) return;
+    const int n = min(4, J.w - x);
+    const uint8_t* src = J.src + (size_t)y * J.src_pitch;
+    uint8_t* dst = J.dst + (size_t)y * J.dst_pitch + x;
+    unsigned l[4] = {0u, 0u, 0u, 0u};
+    const int f = J.format;
+    const int bpp = f == SIFT_PIXEL_RGB8 ? 3 : f == SIFT_PIXEL_LUMAA8 ? 2 : f == SIFT_PIXEL_GRAY8 ? 1 : 4;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (k >= n) break;
+        const uint8_t* p = src + (size_t)(x + k) * bpp;
+        if (bpp >= 3) {
+            const unsigned r = p[0], g = p[1], b = p[2];
+            l[k] = f == SIFT_PIXEL_RGBA8_CANVAS ? luma_canvas(r, g, b) : luma_srgb(r, g, b);
+        } else {
+            l[k] = p[0];
+        }
+    }
+    if (n == 4 && (((uintptr_t)dst) & 3) == 0) {
+        *(unsigned*)dst = l[0] | (l[1] << 8) | (l[2] << 16) | (l[3] << 24);
+    } else {
+        for (int k = 0; k < n; k++) dst[k] = (uint8_t)l[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Lanczos3 resize (image 0.25.6 sample::resize): vertical pass into f32 rows
 // (t += p * w in window order, unrounded), then horizontal pass with
 // round(clamp(t, 0, 255)). Exact reference order (separate mul and add).
@@ -617,6 +667,11 @@ void launch_blur_v_generic(const BlurJob* d_jobs, int njobs, int total_tiles, hi
 void launch_copy_level(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s) {
     if (total_tiles <= 0) return;
     hipLaunchKernelGGL(k_copy_level, dim3(total_tiles), dim3(kThreads), 0, s, d_jobs, njobs);
+}
+
+void launch_luma(const LumaJob* d_jobs, int njobs, int total_tiles, hipStream_t s) {
+    if (total_tiles <= 0) return;
+    hipLaunchKernelGGL(k_luma, dim3(total_tiles), dim3(kThreads), 0, s, d_jobs, njobs);
 }
 
 void launch_resize_v(const ResizeJob* d_jobs, int njobs, int total_tiles, hipStream_t s) {

@@ -40,9 +40,10 @@ constexpr size_t kMaxBatchPixels = (size_t)4200000;
 
 // Device bytes one frame needs in a batch (pyramid, temporaries, detection
 // buffers at the default capacity), used for admission against the budget.
-size_t frame_device_bytes(uint32_t w, uint32_t h, const sift_params_t& p) {
+size_t frame_device_bytes(uint32_t w, uint32_t h, const sift_params_t& p, int bpp = 1) {
     const size_t L = p.num_intervals + 3;
     size_t bytes = round_up_sz(w, 64) * h;  // input copy
+    if (bpp > 1) bytes += round_up_sz((size_t)w * bpp, 64) * h;  // colour pixels
     size_t px_layers = 0;
     uint32_t ww = w, hh = h;
     for (uint32_t o = 0; o < p.num_octaves; o++) {
@@ -58,11 +59,10 @@ size_t frame_device_bytes(uint32_t w, uint32_t h, const sift_params_t& p) {
     return bytes;
 }
 
-int run_and_collect(Context* C, const Batch& B, sift_result_t* out);
-
 struct Request {
     uint64_t tag;
     uint32_t w, h;
+    uint32_t format;  // SIFT_PIXEL_*
     sift_params_t params;
     std::vector<uint8_t> pixels;
     double submit_ms;
@@ -114,9 +114,12 @@ void Server::worker(Context* lane) {
         const double start = now_ms();
         Batch B;
         B.images.resize(batch.size());
+        B.formats.resize(batch.size());
         B.params.resize(batch.size());
         for (size_t i = 0; i < batch.size(); i++) {
-            B.images[i] = sift_image_t{batch[i].pixels.data(), batch[i].w, batch[i].h, batch[i].w};
+            const uint32_t row = batch[i].w * (uint32_t)pixel_bytes(batch[i].format);
+            B.images[i] = sift_image_t{batch[i].pixels.data(), batch[i].w, batch[i].h, row};
+            B.formats[i] = batch[i].format;
             B.params[i] = batch[i].params;
         }
         std::vector<sift_result_t> res(batch.size());
@@ -145,6 +148,46 @@ void Server::worker(Context* lane) {
         cv_done.notify_all();
         cv_work.notify_all();
     }
+}
+
+// Copies the frame into host memory and queues it (sift_hip_submit*).
+int submit_frame(sift_hip_ctx* ctx, const sift_color_image_t* image, const sift_params_t* params,
+                 uint64_t tag) {
+    const double t_submit = now_ms();
+    const int bpp = image ? pixel_bytes(image->format) : 0;
+    if (!ctx || !image || !image->data || bpp == 0 || image->width == 0 || image->height == 0 ||
+        image->width > 65536 || image->height > 65536 ||
+        (image->stride && image->stride < (size_t)image->width * bpp))
+        return SIFT_ERR_INVALID;
+    Context* C = reinterpret_cast<Context*>(ctx);
+    Server* S = C->server;
+    if (!S) return SIFT_ERR_INVALID;
+    const sift_params_t& P = params ? *params : C->params;
+    if (P.num_intervals < 1 || P.num_intervals > 13 || P.num_octaves < 1 || P.num_octaves > 16 ||
+        !(P.sigma > 0.0f))
+        return SIFT_ERR_INVALID;
+    Request r;
+    r.tag = tag;
+    r.w = image->width;
+    r.h = image->height;
+    r.format = image->format;
+    r.params = P;
+    r.bytes = frame_device_bytes(r.w, r.h, P, bpp);
+    if (r.bytes > S->cfg.hbm_budget_bytes) return SIFT_ERR_NOMEM;
+    const size_t row = (size_t)r.w * bpp;
+    r.pixels.resize(row * r.h);
+    const size_t stride = image->stride ? image->stride : row;
+    for (uint32_t y = 0; y < r.h; y++)
+        memcpy(r.pixels.data() + (size_t)y * row, image->data + (size_t)y * stride, row);
+    r.submit_ms = t_submit;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->stopping) return SIFT_ERR_INVALID;
+        if (S->pending.size() >= S->cfg.max_queue) return SIFT_ERR_BUSY;
+        S->pending.push_back(std::move(r));
+    }
+    S->cv_work.notify_one();
+    return SIFT_OK;
 }
 
 }  // namespace sh
@@ -203,38 +246,14 @@ int sift_hip_server_start(sift_hip_ctx* ctx, const sift_server_config_t* cfg) {
 
 int sift_hip_submit(sift_hip_ctx* ctx, const sift_image_t* image, const sift_params_t* params,
                     uint64_t tag) {
-    const double t_submit = now_ms();
-    if (!ctx || !image || !image->data || image->width == 0 || image->height == 0 ||
-        image->width > 65536 || image->height > 65536 ||
-        (image->stride && image->stride < image->width))
-        return SIFT_ERR_INVALID;
-    Context* C = reinterpret_cast<Context*>(ctx);
-    Server* S = C->server;
-    if (!S) return SIFT_ERR_INVALID;
-    const sift_params_t& P = params ? *params : C->params;
-    if (P.num_intervals < 1 || P.num_intervals > 13 || P.num_octaves < 1 || P.num_octaves > 16 ||
-        !(P.sigma > 0.0f))
-        return SIFT_ERR_INVALID;
-    Request r;
-    r.tag = tag;
-    r.w = image->width;
-    r.h = image->height;
-    r.params = P;
-    r.bytes = frame_device_bytes(r.w, r.h, P);
-    if (r.bytes > S->cfg.hbm_budget_bytes) return SIFT_ERR_NOMEM;
-    r.pixels.resize((size_t)r.w * r.h);
-    const size_t stride = image->stride ? image->stride : image->width;
-    for (uint32_t y = 0; y < r.h; y++)
-        memcpy(r.pixels.data() + (size_t)y * r.w, image->data + (size_t)y * stride, r.w);
-    r.submit_ms = t_submit;
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        if (S->stopping) return SIFT_ERR_INVALID;
-        if (S->pending.size() >= S->cfg.max_queue) return SIFT_ERR_BUSY;
-        S->pending.push_back(std::move(r));
-    }
-    S->cv_work.notify_one();
-    return SIFT_OK;
+    if (!image) return SIFT_ERR_INVALID;
+    const sift_color_image_t c{image->data, image->width, image->height, image->stride, SIFT_PIXEL_GRAY8};
+    return sh::submit_frame(ctx, &c, params, tag);
+}
+
+int sift_hip_submit_color(sift_hip_ctx* ctx, const sift_color_image_t* image,
+                          const sift_params_t* params, uint64_t tag) {
+    return sh::submit_frame(ctx, image, params, tag);
 }
 
 int sift_hip_poll(sift_hip_ctx* ctx, sift_completion_t* out, size_t cap, size_t* n, int timeout_ms) {

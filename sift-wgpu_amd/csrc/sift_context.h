@@ -20,6 +20,18 @@
 
 namespace sh {
 
+// Bytes per pixel of a SIFT_PIXEL_* format (0 = unknown format).
+inline int pixel_bytes(uint32_t format) {
+    switch (format) {
+        case SIFT_PIXEL_GRAY8: return 1;
+        case SIFT_PIXEL_RGB8: return 3;
+        case SIFT_PIXEL_RGBA8: return 4;
+        case SIFT_PIXEL_LUMAA8: return 2;
+        case SIFT_PIXEL_RGBA8_CANVAS: return 4;
+        default: return 0;
+    }
+}
+
 struct TapsRef {
     int ntaps = 0, npad = 0;
     float eps_rel = 0.0f;
@@ -61,16 +73,20 @@ struct TableStore {
 
 struct Batch {
     std::vector<sift_image_t> images;
+    std::vector<uint32_t> formats;  // SIFT_PIXEL_* per image; empty = all grey
     std::vector<sift_params_t> params;
+    bool desc_u8 = false;           // also produce the u8 descriptors (RunOutputs::d_desc_u8)
     bool device_input = false;
     bool pyramid_only = false;
     bool want_dog = false;
     bool want_stages = false;
+    uint32_t format(size_t i) const { return formats.empty() ? SIFT_PIXEL_GRAY8 : formats[i]; }
 };
 
 struct RunOutputs {
     Keypoint* d_kps = nullptr;
     float* d_desc = nullptr;
+    uint8_t* d_desc_u8 = nullptr;
     unsigned* d_summary = nullptr;
     Cand* d_cand = nullptr;
     float* d_cand_resp = nullptr;
@@ -151,5 +167,11 @@ struct Context {
     // for table uploads); outputs stay in the arena until the next run.
     int run(const Batch& B, RunOutputs* outs);
 };
+
+// Runs a host-input batch and copies the results out (f32 descriptors into
+// `out`, or u8 descriptors into `out8` when the batch has desc_u8 set), growing
+// the capacities and re-running when a count overflowed them.
+int run_and_collect(Context* C, const Batch& B, sift_result_t* out, sift_result_u8_t* out8 = nullptr,
+                    uint32_t kp_conv = SIFT_KEYPOINTS_CPU);
 
 }  // namespace sh

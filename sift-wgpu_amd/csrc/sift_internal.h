@@ -56,6 +56,20 @@ struct ResizeJob {
     int tiles;
 };
 
+// Colour -> grey conversion of one input image (to_luma8 / the wasm canvas rule).
+// Pixel formats are the SIFT_PIXEL_* values of the C ABI.
+struct LumaJob {
+    const uint8_t* src;
+    uint8_t* dst;
+    int w, h;
+    int src_pitch, dst_pitch;
+    int format;
+    int tiles_x;
+    int tile_begin;
+    int tiles;
+};
+constexpr int kLumaTileW = 1024;  // 256 lanes x 4 px
+
 // Per-image parameters read by the detection kernels.
 struct ImageParams {
     float sigma;
@@ -110,6 +124,8 @@ void launch_blur_h_small(const BlurJob* d_jobs, int njobs, int total_tiles, int 
 void launch_blur_v_small(const BlurJob* d_jobs, int njobs, int total_tiles, int max_taps4,
                          int max_rows, hipStream_t s);
 void launch_blur_h_generic(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
+// One tile = kLumaTileW pixels of one row.
+void launch_luma(const LumaJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
 void launch_blur_v_generic(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
 
 void launch_extrema(const OctaveDesc* d_oct, int noct, int total_rows, const ImageParams* d_params,
@@ -141,6 +157,10 @@ void launch_descriptor(const OctaveDesc* d_oct, const ImageParams* d_params, con
 void launch_finalize(const unsigned* d_slot_offsets, const int* d_img_slot_begin, int nimg,
                      const unsigned* d_koff, const unsigned* d_total, int cap,
                      unsigned* d_summary, hipStream_t s);
+// u8 descriptors of the WebGPU backends: out = trunc(clamp(d * 512, 0, 255)) for
+// the first min(*d_count, kcap) descriptors.
+void launch_desc_u8(const float* d_desc, const unsigned* d_count, int kcap, uint8_t* d_out,
+                    hipStream_t s);
 // Max padded tap count handled by the LDS row-tile horizontal pass.
 constexpr int kMaxHPad = 3576;
 // Small-level kernels: taps (rounded up to 4) and staged rows they accept.

@@ -5,6 +5,9 @@ Mirrors (stevel705/sift-wgpu):
   Sift.detect_and_compute_with_backend src/sift.rs:1244-1355
   SiftBackend + from_str aliases       src/lib.rs:18-49 (plus "hip")
   KeyPoint                             src/keypoints.rs:1-10
+  colour input (to_luma8 on the device) src/sift.rs:1202, src/wasm.rs:79-103
+  u8 descriptors / V2 keypoint view    src/shaders/descriptor.wgsl:256-262,
+                                       src/sift.rs:1288-1291, src/gpu_sift_v2.rs:1699-1707
 
 Only the Hip backend is implemented here; the reference's Cpu / WebGpu arms
 live in the Rust crate and are not reachable from this library, so asking for
@@ -49,6 +52,25 @@ class SiftResult(ctypes.Structure):
                 ("count", ctypes.c_size_t)]
 
 
+class SiftColorImage(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32), ("stride", ctypes.c_uint32),
+                ("format", ctypes.c_uint32)]
+
+
+class SiftResultU8(ctypes.Structure):
+    _fields_ = [("keypoints", ctypes.c_void_p), ("descriptors", ctypes.POINTER(ctypes.c_uint8)),
+                ("count", ctypes.c_size_t)]
+
+
+# pixel formats of SiftColorImage (SIFT_PIXEL_* in the C header)
+PIXEL_GRAY8, PIXEL_RGB8, PIXEL_RGBA8, PIXEL_LUMAA8, PIXEL_RGBA8_CANVAS = 0, 1, 2, 3, 4
+PIXEL_BYTES = {PIXEL_GRAY8: 1, PIXEL_RGB8: 3, PIXEL_RGBA8: 4, PIXEL_LUMAA8: 2,
+               PIXEL_RGBA8_CANVAS: 4}
+# keypoint conventions of detect_batch_u8 (SIFT_KEYPOINTS_* in the C header)
+KEYPOINTS_CPU, KEYPOINTS_WGPU_V2 = 0, 1
+
+
 class SiftStages(ctypes.Structure):
     _fields_ = [("candidates", ctypes.c_void_p), ("n_candidates", ctypes.c_size_t),
                 ("refined", ctypes.c_void_p), ("n_refined", ctypes.c_size_t),
@@ -79,6 +101,8 @@ EXPORTED_SYMBOLS = [
     "sift_hip_detect_stages", "sift_stages_free", "sift_hip_profile", "sift_hip_profile_filter",
     "sift_hip_profile_read",
     "sift_hip_server_start", "sift_hip_submit", "sift_hip_poll", "sift_hip_server_stop",
+    "sift_hip_to_luma8", "sift_hip_detect_color", "sift_hip_detect_batch_color",
+    "sift_hip_detect_batch_u8", "sift_result_u8_free", "sift_hip_submit_color",
 ]
 
 _LIB = None
@@ -135,6 +159,20 @@ def lib():
         L.sift_hip_poll.restype = ctypes.c_int
         L.sift_hip_server_stop.argtypes = [ctypes.c_void_p]
         L.sift_hip_server_stop.restype = ctypes.c_int
+        L.sift_hip_to_luma8.argtypes = [ctypes.c_void_p, P(SiftColorImage), P(ctypes.c_uint8)]
+        L.sift_hip_to_luma8.restype = ctypes.c_int
+        L.sift_hip_detect_color.argtypes = [ctypes.c_void_p, P(SiftColorImage), P(SiftResult)]
+        L.sift_hip_detect_color.restype = ctypes.c_int
+        L.sift_hip_detect_batch_color.argtypes = [ctypes.c_void_p, P(SiftColorImage), P(SiftParams),
+                                                  ctypes.c_size_t, P(SiftResult)]
+        L.sift_hip_detect_batch_color.restype = ctypes.c_int
+        L.sift_hip_detect_batch_u8.argtypes = [ctypes.c_void_p, P(SiftColorImage), P(SiftParams),
+                                               ctypes.c_size_t, ctypes.c_uint32, P(SiftResultU8)]
+        L.sift_hip_detect_batch_u8.restype = ctypes.c_int
+        L.sift_result_u8_free.argtypes = [P(SiftResultU8)]
+        L.sift_hip_submit_color.argtypes = [ctypes.c_void_p, P(SiftColorImage), P(SiftParams),
+                                            ctypes.c_uint64]
+        L.sift_hip_submit_color.restype = ctypes.c_int
         _LIB = L
         return L
 
@@ -207,6 +245,56 @@ def _image_struct(img):
     return img, SiftImage(img.ctypes.data, w, h, w)
 
 
+def _color_struct(img, fmt=None):
+    """(array kept alive, SiftColorImage). 2-D arrays are grey; (H, W, 3) RGB8,
+    (H, W, 4) RGBA8 (or fmt=PIXEL_RGBA8_CANVAS), (H, W, 2) LumaA8."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    if fmt is None:
+        if img.ndim == 2:
+            fmt = PIXEL_GRAY8
+        elif img.ndim == 3 and img.shape[2] in (2, 3, 4):
+            fmt = {2: PIXEL_LUMAA8, 3: PIXEL_RGB8, 4: PIXEL_RGBA8}[img.shape[2]]
+        else:
+            raise ValueError("expected (H, W) grey or (H, W, 2|3|4) u8 pixels")
+    if fmt not in PIXEL_BYTES:
+        raise ValueError(f"unknown pixel format {fmt}")
+    h, w = img.shape[:2]
+    if img.size != h * w * PIXEL_BYTES[fmt]:
+        raise ValueError("array shape does not match the pixel format")
+    return img, SiftColorImage(img.ctypes.data, w, h, w * PIXEL_BYTES[fmt], fmt)
+
+
+def to_luma8(img, fmt=None, device=0):
+    """Grey u8 image of a colour image, converted on the device (to_luma8,
+    src/sift.rs:1202; PIXEL_RGBA8_CANVAS: src/wasm.rs:79-94)."""
+    ctx = _Ctx.get(device)
+    a, s = _color_struct(img, fmt)
+    out = np.empty((s.height, s.width), np.uint8)
+    _check(lib().sift_hip_to_luma8(ctx, ctypes.byref(s),
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))),
+           "sift_hip_to_luma8")
+    return out
+
+
+def wgpu_descriptor_view(desc_u8):
+    """The f32 form the reference's WebGPU arms hand to callers: u8 / 255
+    (src/sift.rs:1288-1291)."""
+    return np.asarray(desc_u8, np.uint8).astype(np.float32) / np.float32(255.0)
+
+
+def _take_result_u8(r):
+    n = int(r.count)
+    if n:
+        kb = (ctypes.c_char * (n * KEYPOINT_DTYPE.itemsize)).from_address(r.keypoints)
+        kps = np.frombuffer(bytes(kb), KEYPOINT_DTYPE).copy()
+        desc = np.ctypeslib.as_array(r.descriptors, shape=(n * 128,)).copy().reshape(n, 128)
+    else:
+        kps = np.zeros(0, KEYPOINT_DTYPE)
+        desc = np.zeros((0, 128), np.uint8)
+    lib().sift_result_u8_free(ctypes.byref(r))
+    return kps, desc
+
+
 def _take_result(r):
     n = int(r.count)
     if n:
@@ -276,27 +364,56 @@ class Sift:
         return self.detect_and_compute_with_backend(img, SiftBackend.Hip)
 
     # -- batched / device / introspection ------------------------------------
-    def detect_batch(self, images, params=None):
+    def _params_array(self, n, params):
+        pa = (SiftParams * n)()
+        for i in range(n):
+            p = params[i] if params is not None else self
+            pa[i] = p.params() if isinstance(p, Sift) else p
+        return pa
+
+    def detect_batch(self, images, params=None, formats=None):
+        """images: grey (H, W) arrays, or colour arrays (see _color_struct; formats
+        may name PIXEL_RGBA8_CANVAS per image), converted to grey on the device."""
         ctx = _Ctx.get(self.device)
+        n = len(images)
+        pa = self._params_array(n, params)
+        outs = (SiftResult * n)()
         keep = []
-        ims = (SiftImage * len(images))()
+        if formats is None and all(np.ndim(im) == 2 for im in images):
+            ims = (SiftImage * n)()
+            for i, im in enumerate(images):
+                a, s = _image_struct(im)
+                keep.append(a)
+                ims[i] = s
+            _check(lib().sift_hip_detect_batch(ctx, ims, pa, n, outs), "sift_hip_detect_batch")
+        else:
+            ims = (SiftColorImage * n)()
+            for i, im in enumerate(images):
+                a, s = _color_struct(im, None if formats is None else formats[i])
+                keep.append(a)
+                ims[i] = s
+            _check(lib().sift_hip_detect_batch_color(ctx, ims, pa, n, outs),
+                   "sift_hip_detect_batch_color")
+        return [_take_result(outs[i]) for i in range(n)]
+
+    def detect_batch_u8(self, images, params=None, keypoint_convention=KEYPOINTS_CPU,
+                        formats=None):
+        """Keypoints and u8 descriptors as the reference's WebGPU backends produce
+        them (src/shaders/descriptor.wgsl:256-262); KEYPOINTS_WGPU_V2 reports the
+        keypoints as the V2 backend does (src/gpu_sift_v2.rs:1699-1707)."""
+        ctx = _Ctx.get(self.device)
+        n = len(images)
+        pa = self._params_array(n, params)
+        ims = (SiftColorImage * n)()
+        keep = []
         for i, im in enumerate(images):
-            a, s = _image_struct(im)
+            a, s = _color_struct(im, None if formats is None else formats[i])
             keep.append(a)
             ims[i] = s
-        pa = None
-        if params is not None:
-            pa = (SiftParams * len(images))()
-            for i, p in enumerate(params):
-                pa[i] = p.params() if isinstance(p, Sift) else p
-        else:
-            pa = (SiftParams * len(images))()
-            for i in range(len(images)):
-                pa[i] = self.params()
-        outs = (SiftResult * len(images))()
-        _check(lib().sift_hip_detect_batch(ctx, ims, pa, len(images), outs),
-               "sift_hip_detect_batch")
-        return [_take_result(outs[i]) for i in range(len(images))]
+        outs = (SiftResultU8 * n)()
+        _check(lib().sift_hip_detect_batch_u8(ctx, ims, pa, n, keypoint_convention, outs),
+               "sift_hip_detect_batch_u8")
+        return [_take_result_u8(outs[i]) for i in range(n)]
 
     def detect_batch_device(self, images, params=None):
         """images: list of (device_ptr, w, h, stride). Returns (counts, kp_ptr,
@@ -424,14 +541,19 @@ class SiftServer:
             raise SiftError(f"sift_hip_server_start failed ({rc})")
         self._keep = {}
 
-    def submit(self, img, tag, sift=None):
+    def submit(self, img, tag, sift=None, fmt=None):
         """Queues a frame (copied before returning); `sift` optionally gives this
-        frame's parameters (a Sift), else the server's."""
-        img = np.ascontiguousarray(img, dtype=np.uint8)
-        h, w = img.shape
-        s = SiftImage(img.ctypes.data, w, h, w)
+        frame's parameters (a Sift), else the server's. Colour frames ((H, W, C)
+        arrays, or fmt) are converted to grey on the device."""
         pp = ctypes.byref(sift.params()) if sift is not None else None
-        rc = lib().sift_hip_submit(self._ctx, ctypes.byref(s), pp, tag)
+        if fmt is None and np.ndim(img) == 2:
+            img = np.ascontiguousarray(img, dtype=np.uint8)
+            h, w = img.shape
+            s = SiftImage(img.ctypes.data, w, h, w)
+            rc = lib().sift_hip_submit(self._ctx, ctypes.byref(s), pp, tag)
+        else:
+            img, s = _color_struct(img, fmt)
+            rc = lib().sift_hip_submit_color(self._ctx, ctypes.byref(s), pp, tag)
         if rc != SIFT_OK:
             raise SiftError(f"sift_hip_submit failed ({rc})")
 
