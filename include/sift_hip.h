@@ -17,6 +17,7 @@
  *                           stage outputs                  src/sift.rs:510-603,299-474
  *   sift_hip_to_luma8,    <- DynamicImage::to_luma8 before the CPU path src/sift.rs:1202,
  *   sift_hip_detect*_color   img_from_raw of the wasm entry          src/wasm.rs:79-103
+ *   sift_draw_keypoints  <- draw_keypoints_to_image                  src/lib.rs:70-116
  *   sift_hip_detect_batch_u8 <- the WebGPU arms' u8 descriptors and V2 keypoints
  *                           src/shaders/descriptor.wgsl:256-262, src/sift.rs:1288-1291,
  *                           src/gpu_sift_v2.rs:1699-1707
@@ -225,6 +226,14 @@ int sift_hip_detect_batch_u8(sift_hip_ctx* ctx, const sift_color_image_t* images
                              const sift_params_t* params, size_t n, uint32_t keypoint_convention,
                              sift_result_u8_t* out);
 void sift_result_u8_free(sift_result_u8_t* r);
+
+/* draw_keypoints_to_image (src/lib.rs:70-116), on the host (no device work):
+ * the image as RGB8 (DynamicImage::to_rgb8), then per keypoint a filled circle
+ * of radius 6 at (x as i32, y as i32), a line of length 6 along its angle and a
+ * filled circle of radius 1, in `color`, rasterised as imageproc 0.25.0's
+ * draw_filled_circle_mut / draw_line_segment_mut. out_rgb: width*height*3 bytes. */
+int sift_draw_keypoints(const sift_color_image_t* image, const sift_keypoint_t* keypoints,
+                        size_t count, const uint8_t color[3], uint8_t* out_rgb);
 
 int sift_hip_server_start(sift_hip_ctx* ctx, const sift_server_config_t* cfg);
 /* Copies the pixels before returning. params: NULL = the context's params,

@@ -73,6 +73,9 @@ def lib():
         L.oracle_to_luma8.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
                                       u8p]
         L.oracle_descriptor_u8.argtypes = [f32p, ctypes.c_size_t, u8p]
+        L.oracle_draw_keypoints.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
+                                            ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, u8p,
+                                            u8p]
         _LIB = L
     return _LIB
 
@@ -221,6 +224,18 @@ def to_luma8(img, fmt):
         raise ValueError("array shape does not match the pixel format")
     out = np.empty((h, w), np.uint8)
     lib().oracle_to_luma8(_u8(img), w, h, w * _BPP[fmt], fmt, _u8(out))
+    return out
+
+
+def draw_keypoints(img, fmt, kps, color=(255, 0, 0)):
+    """draw_keypoints_to_image (src/lib.rs:70-116) of an (H, W[, C]) u8 image."""
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape[:2]
+    out = np.empty((h, w, 3), np.uint8)
+    kps = np.ascontiguousarray(kps)
+    col = np.array(color, np.uint8)
+    lib().oracle_draw_keypoints(_u8(img), w, h, w * _BPP[fmt], fmt,
+                                kps.ctypes.data if len(kps) else None, len(kps), _u8(col), _u8(out))
     return out
 
 

@@ -33,6 +33,13 @@
  *    Rec. 709 value. The integer weights themselves are the crate's published
  *    constants; that the crate truncates (not rounds) is UNPINNED by any
  *    reference test.
+ *  - draw_keypoints_to_image (src/lib.rs:70-116; imageproc 0.25.0 drawing,
+ *    source not available here): the radius-1 circle is the 5-pixel cross, the
+ *    radius-6 circle is symmetric under both reflections and the transposition
+ *    and lies between the discs of radius 5.5 and 6.5, lines stay within 1.5 px
+ *    of the segment and reach its far end, the 45-degree line from an integer
+ *    point is the diagonal, and points outside the image are skipped. The exact pixel choice of the crate
+ *    is UNPINNED by any reference test.
  *  - the wasm canvas conversion (src/wasm.rs:88-94, in the reference itself):
  *    all 2^24 triples against exact rational arithmetic on the f32 constants,
  *    which fixes every result except where the f32 sum rounds across an integer.
@@ -925,5 +932,90 @@ void oracle_descriptor_u8(const float* desc, size_t n, uint8_t* out) {
         float v = desc[i] * 512.0f;
         v = v < 0.0f ? 0.0f : (v > 255.0f ? 255.0f : v);
         out[i] = (uint8_t)v;
+    }
+}
+
+/* imageproc 0.25.0 drawing: BresenhamLineIter over f32 end points (steep
+ * lines swap axes, the walk runs from the smaller x; error starts at dx/2 and
+ * steps y when it goes negative), points outside the canvas skipped;
+ * draw_filled_circle_mut: midpoint circle (p = 1 - r), four horizontal
+ * segments per step. */
+static void draw_px(uint8_t* rgb, int w, int h, const uint8_t c[3], int x, int y) {
+    if (x < 0 || y < 0 || x >= w || y >= h) return;
+    uint8_t* p = rgb + ((size_t)y * w + x) * 3;
+    p[0] = c[0];
+    p[1] = c[1];
+    p[2] = c[2];
+}
+
+static void draw_line(uint8_t* rgb, int w, int h, const uint8_t c[3], float x0, float y0, float x1,
+                      float y1) {
+    int steep = fabsf(y1 - y0) > fabsf(x1 - x0);
+    float t;
+    if (steep) {
+        t = x0; x0 = y0; y0 = t;
+        t = x1; x1 = y1; y1 = t;
+    }
+    if (x0 > x1) {
+        t = x0; x0 = x1; x1 = t;
+        t = y0; y0 = y1; y1 = t;
+    }
+    float dx = x1 - x0, dy = fabsf(y1 - y0), err = dx / 2.0f;
+    int x = f2i_sat(x0), y = f2i_sat(y0), end_x = f2i_sat(x1), ystep = y0 < y1 ? 1 : -1;
+    while (x <= end_x) {
+        if (steep) draw_px(rgb, w, h, c, y, x);
+        else draw_px(rgb, w, h, c, x, y);
+        x++;
+        err -= dy;
+        if (err < 0.0f) {
+            y += ystep;
+            err += dx;
+        }
+    }
+}
+
+static void draw_disc(uint8_t* rgb, int w, int h, const uint8_t c[3], int x0, int y0, int r) {
+    int x = 0, y = r, p = 1 - r;
+    while (x <= y) {
+        draw_line(rgb, w, h, c, (float)(x0 - x), (float)(y0 + y), (float)(x0 + x), (float)(y0 + y));
+        draw_line(rgb, w, h, c, (float)(x0 - y), (float)(y0 + x), (float)(x0 + y), (float)(y0 + x));
+        draw_line(rgb, w, h, c, (float)(x0 - x), (float)(y0 - y), (float)(x0 + x), (float)(y0 - y));
+        draw_line(rgb, w, h, c, (float)(x0 - y), (float)(y0 - x), (float)(x0 + y), (float)(y0 - x));
+        x++;
+        if (p < 0) {
+            p += 2 * x + 1;
+        } else {
+            y--;
+            p += 2 * (x - y) + 1;
+        }
+    }
+}
+
+void oracle_draw_keypoints(const uint8_t* src, int w, int h, size_t stride, int format,
+                           const oracle_keypoint_t* kps, size_t n, const uint8_t c[3],
+                           uint8_t* rgb) {
+    const int bpp = format == 1 ? 3 : format == 3 ? 2 : format == 0 ? 1 : 4;
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            const uint8_t* p = src + (size_t)y * stride + (size_t)x * bpp;
+            uint8_t* d = rgb + ((size_t)y * w + x) * 3;
+            if (bpp <= 2) {
+                d[0] = d[1] = d[2] = p[0];
+            } else {
+                d[0] = p[0];
+                d[1] = p[1];
+                d[2] = p[2];
+            }
+        }
+    for (size_t i = 0; i < n; i++) {
+        const float x = kps[i].x, y = kps[i].y;
+        const float size = 2.0f; /* src/lib.rs:87 */
+        int radius = f2i_sat(roundf(size * 3.0f));
+        int disp = radius > 2 ? radius : 2;
+        draw_disc(rgb, w, h, c, f2i_sat(x), f2i_sat(y), disp);
+        float xe = x + (float)disp * cosf(kps[i].angle);
+        float ye = y + (float)disp * sinf(kps[i].angle);
+        draw_line(rgb, w, h, c, x, y, xe, ye);
+        draw_disc(rgb, w, h, c, f2i_sat(x), f2i_sat(y), 1);
     }
 }

@@ -97,6 +97,14 @@ void oracle_free(void* ptr);
  * (src/wasm.rs:79-94). stride = bytes between input rows. */
 void oracle_to_luma8(const uint8_t* src, int w, int h, size_t stride, int format, uint8_t* dst);
 
+/* draw_keypoints_to_image (src/lib.rs:70-116): rgb (w*h*3) = to_rgb8 of the
+ * input (format as for oracle_to_luma8), then per keypoint imageproc 0.25.0's
+ * draw_filled_circle_mut (radius 6), draw_line_segment_mut (length 6 along
+ * the angle) and draw_filled_circle_mut (radius 1) in colour c. */
+void oracle_draw_keypoints(const uint8_t* src, int w, int h, size_t stride, int format,
+                           const oracle_keypoint_t* kps, size_t n, const uint8_t c[3],
+                           uint8_t* rgb);
+
 /* u8 descriptors of the WebGPU backends (src/shaders/descriptor.wgsl:256-262):
  * u32(clamp(d * 512, 0, 255)) per value, n values. */
 void oracle_descriptor_u8(const float* desc, size_t n, uint8_t* out);

@@ -103,6 +103,7 @@ EXPORTED_SYMBOLS = [
     "sift_hip_server_start", "sift_hip_submit", "sift_hip_poll", "sift_hip_server_stop",
     "sift_hip_to_luma8", "sift_hip_detect_color", "sift_hip_detect_batch_color",
     "sift_hip_detect_batch_u8", "sift_result_u8_free", "sift_hip_submit_color",
+    "sift_draw_keypoints",
 ]
 
 _LIB = None
@@ -173,6 +174,9 @@ def lib():
         L.sift_hip_submit_color.argtypes = [ctypes.c_void_p, P(SiftColorImage), P(SiftParams),
                                             ctypes.c_uint64]
         L.sift_hip_submit_color.restype = ctypes.c_int
+        L.sift_draw_keypoints.argtypes = [P(SiftColorImage), ctypes.c_void_p, ctypes.c_size_t,
+                                          P(ctypes.c_uint8), P(ctypes.c_uint8)]
+        L.sift_draw_keypoints.restype = ctypes.c_int
         _LIB = L
         return L
 
@@ -273,6 +277,20 @@ def to_luma8(img, fmt=None, device=0):
     _check(lib().sift_hip_to_luma8(ctx, ctypes.byref(s),
                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))),
            "sift_hip_to_luma8")
+    return out
+
+
+def draw_keypoints_to_image(img, keypoints, color=(255, 0, 0), fmt=None):
+    """draw_keypoints_to_image (src/lib.rs:70-116): an (H, W, 3) u8 RGB image with
+    every keypoint drawn in `color` (host code of the library, no device work)."""
+    a, s = _color_struct(img, fmt)
+    kps = np.ascontiguousarray(keypoints, KEYPOINT_DTYPE)
+    out = np.empty((s.height, s.width, 3), np.uint8)
+    col = (ctypes.c_uint8 * 3)(*color)
+    _check(lib().sift_draw_keypoints(ctypes.byref(s), kps.ctypes.data if len(kps) else None,
+                                     len(kps), col,
+                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))),
+           "sift_draw_keypoints")
     return out
 
 
