@@ -960,10 +960,10 @@ __device__ __forceinline__ void desc_finish(const unsigned* hist, float fix, int
 // and bilinear(x_s, y_s +- 1)).
 // Per peak, the samples of each row whose rotated bin coordinates can fall in
 // (-1, 4) form one interval (the rotated window is convex); the intervals,
-// widened by one sample and clipped to the in-image rectangle, are laid end to
-// end and split into 256 contiguous chunks, one per thread, so nearly every lane
-// works on an in-window sample and the lanes of a wave sit far apart in the
-// window (different histogram cells). Each sample still applies the window
+// widened by one sample and clipped to the in-image rectangle, are cut into at
+// most 256 segments of at most `per` consecutive samples of one row, one per
+// thread, so nearly every lane works on in-window samples of a single row and
+// the lanes of a wave sit far apart in the window (different histogram cells). Each sample still applies the window
 // test xb, yb in (-1, 4), then computes mag * exp(-(px^2+py^2)/(2 (6 sigma_o)^2))
 // and atan2 and scatters trilinearly into its copy of the histogram (even and
 // odd lanes use separate copies, which halves same-address collisions of the
@@ -986,6 +986,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
     __shared__ float4 rrec[64];  // per row: first and last column (int bits), sin * py, cos * py
     __shared__ float rwy[64];
     __shared__ int rpre[65];
+    __shared__ int seg_len;
     __shared__ float pang[kMaxPeaks], pcos[kMaxPeaks], psin[kMaxPeaks];
     uint8_t* const patch = reinterpret_cast<uint8_t*>(pu);
     unsigned* const hist = pu;
@@ -1094,30 +1095,41 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                                             sa * py * inv_bw, ca * py * inv_bw);
                     rwy[tid] = exs[cy] * kDescFix;
                 }
-                rpre[tid + 1] = wave_incl_scan(cnt);
-                if (tid == 0) rpre[0] = 0;
+                // Each row is cut into segments of at most `per` consecutive samples,
+                // one segment per thread: per is the smallest value >= T / 256 that
+                // gives at most 256 segments (per <= 16: a row holds at most 63
+                // samples, so per = 16 gives at most 4 * 63 segments).
+                const int T = __builtin_amdgcn_readlane(wave_incl_scan(cnt), 63);
+                int per = max((T + kThreads - 1) / kThreads, 1);
+                int segs = wave_incl_scan((cnt + per - 1) / per);
+                while (__builtin_amdgcn_readlane(segs, 63) > kThreads) {
+                    per++;
+                    segs = wave_incl_scan((cnt + per - 1) / per);
+                }
+                rpre[tid + 1] = segs;  // segment prefix over the rows
+                if (tid == 0) {
+                    rpre[0] = 0;
+                    seg_len = per;
+                }
             }
             __syncthreads();
-            const int T = rpre[nrows];
-            const int per = (T + kThreads - 1) / kThreads;  // <= 16
-            int k = (lane * 4 + wave) * per;
-            const int kend = min(k + per, T);
-            if (k < kend) {
+            const int per = seg_len;
+            const int sg = lane * 4 + wave;  // lanes of a wave take segments far apart
+            if (sg < rpre[nrows]) {
                 unsigned* hw = hist + (lane & 1) * kHistWords;  // neighbouring lanes: different copies
-                int r0 = 0, r1 = nrows;  // rpre[r0] <= k < rpre[r1]
+                int r0 = 0, r1 = nrows;  // rpre[r0] <= sg < rpre[r1]: row r0 holds the segment
                 while (r1 - r0 > 1) {
                     const int mid = (r0 + r1) >> 1;
-                    if (rpre[mid] <= k) r0 = mid;
+                    if (rpre[mid] <= sg) r0 = mid;
                     else r1 = mid;
                 }
-                int r = r0;
-                float4 rr = rrec[r];
-                int cx = __float_as_int(rr.x) + (k - rpre[r]);
-                int cxe = __float_as_int(rr.y);
-                float wyf = rwy[r];
+                const float4 rr = rrec[r0];
+                int cx = __float_as_int(rr.x) + (sg - rpre[r0]) * per;
+                const int len = min(per, __float_as_int(rr.y) - cx + 1);
+                const float wyf = rwy[r0];
                 // grid point (cx, cy) of sample (cx, cy) -- the up-left neighbour -- so that
                 // all four gradient neighbours sit at non-negative offsets
-                const float* grow = grid + (by0 + r) * kGS;
+                const float* grow = grid + (by0 + r0) * kGS;
                 // rotated bin coordinates (rx / bw, ry / bw) (src/sift.rs:967-968), advanced
                 // by one sample step along the row
                 const float cai = ca * inv_bw, msai = msa * inv_bw;
@@ -1127,19 +1139,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                 float acc[8];
                 float u = __builtin_fmaf(cai, (float)(cx - R), rr.z);
                 float v = __builtin_fmaf(msai, (float)(cx - R), rr.w);
-                for (; k < kend; k++, cx++, u += cai, v += msai) {
-                    if (cx > cxe) {  // next non-empty row (one exists: k < T)
-                        do {
-                            r++;
-                        } while (r < nrows - 1 && rpre[r + 1] == rpre[r]);
-                        rr = rrec[r];
-                        cx = __float_as_int(rr.x);
-                        cxe = __float_as_int(rr.y);
-                        wyf = rwy[r];
-                        grow = grid + (by0 + r) * kGS;
-                        u = __builtin_fmaf(cai, (float)(cx - R), rr.z);
-                        v = __builtin_fmaf(msai, (float)(cx - R), rr.w);
-                    }
+                for (int i = 0; i < len; i++, cx++, u += cai, v += msai) {
                     // window: bin coordinates u + 1.5, v + 1.5 in (-1, 4) (src/sift.rs:969-977)
                     if (!(fabsf(u) < 2.5f && fabsf(v) < 2.5f)) continue;
                     const float* gq = grow + cx;
