@@ -49,8 +49,9 @@ __device__ __forceinline__ float q255(unsigned v) { return __fdiv_rn((float)v, 2
 // rule), with one pad word after every 32 (element c at c + c/32) so that the
 // lanes' reads, 8 elements apart, fall in distinct banks. A register window of
 // PX+4 inputs slides over the taps in blocks of 4; taps are zero-padded to a
-// multiple of 4 (p * 0 added to a non-negative sum leaves it unchanged) and
-// are wave-uniform (scalar registers).
+// multiple of 4 (p * 0 added to a non-negative sum leaves it unchanged), except
+// that a single leftover tap is applied on its own, and are wave-uniform
+// (scalar registers).
 // ---------------------------------------------------------------------------
 __host__ __device__ __forceinline__ int hpad(int c) { return c + (c >> 5); }
 
@@ -107,7 +108,12 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
     const float* row = lds + wave * rowstride;
     const int base = lane * PX;
     const float* __restrict__ taps = J.taps;
-    const int n4 = (J.ntaps + 3) & ~3;  // <= ntaps_pad; taps beyond ntaps are zero
+    // Taps in blocks of 4 (taps beyond ntaps are zero, <= ntaps_pad). When one tap
+    // is left over (ntaps = 4k + 1), it is applied after the blocks from the window
+    // registers, which then hold row[base + 4k ..], instead of a block of three
+    // zero taps.
+    const bool one_left = (J.ntaps & 3) == 1;
+    const int nb = one_left ? (J.ntaps & ~3) : ((J.ntaps + 3) & ~3);
 
     float acc[PX];
 #pragma unroll
@@ -115,7 +121,7 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
     float win[PX + 4];
 #pragma unroll
     for (int k = 0; k < PX; k++) win[k] = row[hpad(base + k)];
-    for (int b = 0; b < n4; b += 4) {
+    for (int b = 0; b < nb; b += 4) {
 #pragma unroll
         for (int k = 0; k < 4; k++) win[PX + k] = row[hpad(base + PX + b + k)];
 #pragma unroll
@@ -126,6 +132,11 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
         }
 #pragma unroll
         for (int k = 0; k < PX; k++) win[k] = win[k + 4];
+    }
+    if (one_left) {
+        const float wk = taps[nb];
+#pragma unroll
+        for (int q = 0; q < PX; q++) acc[q] = mul_add_exact(acc[q], win[q], wk);
     }
 
     const int x = x0 + base;
