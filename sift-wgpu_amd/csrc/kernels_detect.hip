@@ -1342,7 +1342,9 @@ void launch_refine(const OctaveDesc* d_oct, const ImageParams* d_params, const C
 void launch_orient(const OctaveDesc* d_oct, const ImageParams* d_params, const Cand* d_cand,
                    const unsigned* d_total, int cap, const Keypoint* d_ref, const int* d_valid,
                    unsigned* d_npeaks, float* d_angles, hipStream_t s) {
-    hipLaunchKernelGGL(k_orient, dim3(grid_for(cap, 1, 16384)), dim3(64), 0, s, d_oct, d_params,
+    // grid-stride over the candidates with many more workgroups than fit at once:
+    // each workgroup takes only a few keypoints, so the launch ends evenly
+    hipLaunchKernelGGL(k_orient, dim3(grid_for(cap, 1, 65536)), dim3(64), 0, s, d_oct, d_params,
                        d_cand, d_total, cap, d_ref, d_valid, d_npeaks, d_angles);
 }
 
@@ -1351,7 +1353,8 @@ void launch_descriptor(const OctaveDesc* d_oct, const ImageParams* d_params, con
                        const unsigned* d_npeaks, const float* d_angles, const unsigned* d_koff,
                        int kcap, Keypoint* d_kps, float* d_desc, unsigned* d_wide_count,
                        int* d_wide_list, hipStream_t s) {
-    hipLaunchKernelGGL(k_descriptor, dim3(grid_for(cap, 1, 8192)), dim3(kThreads), 0, s, d_oct,
+    // as k_orient: few keypoints per workgroup, so the launch ends evenly
+    hipLaunchKernelGGL(k_descriptor, dim3(grid_for(cap, 1, 49152)), dim3(kThreads), 0, s, d_oct,
                        d_params, d_cand, d_total, cap, d_ref, d_npeaks, d_angles, d_koff, kcap,
                        d_kps, d_desc, d_wide_count, d_wide_list);
     hipLaunchKernelGGL(k_descriptor_wide, dim3(256), dim3(kThreads), 0, s, d_oct, d_params, d_cand,
