@@ -86,6 +86,10 @@ __device__ __forceinline__ float atan2_bins(float y, float x) {
     return copysignf(r, y);
 }
 
+// v / 255 in f32 without a table: (float)((double)v * (1/255)) equals the IEEE
+// f32 division for every byte value.
+__device__ __forceinline__ float q255x(unsigned v) { return (float)((double)v * (1.0 / 255.0)); }
+
 __device__ __forceinline__ void fill_q(float* q) {
     for (int i = threadIdx.x; i < 256; i += blockDim.x) q[i] = __fdiv_rn((float)i, 255.0f);
 }
@@ -555,7 +559,7 @@ __global__ __launch_bounds__(kThreads) void k_refine(const OctaveDesc* __restric
 // Gaussian level (every pixel the gradient of a sample can touch) are staged
 // in LDS as q = v/255 floats. Lane l takes samples l, l+64, ... of the
 // (2R+1)^2 window in raster order; each adds mag * exp(-(dx^2+dy^2)/denom) to
-// bin floor(theta * 36 / 2pi) of a u64 fixed-point (2^40) histogram, one of 8
+// bin floor(theta * 36 / 2pi) of a u64 fixed-point (2^40) histogram, one of 4
 // copies by lane (neighbouring lanes, whose samples are neighbours, hit
 // different copies; integer adds: order-independent, deterministic, and as
 // fine as the f32 contributions themselves). Lanes 0..35 then hold one bin
@@ -564,7 +568,7 @@ __global__ __launch_bounds__(kThreads) void k_refine(const OctaveDesc* __restric
 // ---------------------------------------------------------------------------
 constexpr int kOriMaxR = 16;
 constexpr int kOriRows = 2 * kOriMaxR + 5;  // staged rows
-constexpr int kOS = 2 * kOriMaxR + 12;       // staged row stride (floats), >= rows + 6
+constexpr int kOS = 2 * kOriMaxR + 8;       // staged row stride (floats): 4 * ceil((3 + rows + 3) / 4)
 
 __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oct,
                                                const ImageParams* __restrict__ params,
@@ -574,11 +578,10 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                                                const int* __restrict__ valid,
                                                unsigned* __restrict__ npeaks,
                                                float* __restrict__ angles) {
-    __shared__ float q[256];
-    __shared__ unsigned long long hist[8][36];  // 8 copies (lane & 7)
+    // 4 histogram copies (lane & 3) and the staged window: 7 KB of LDS, so 22
+    // work-groups (one wave each) fit on a CU
+    __shared__ unsigned long long hist[4][36];
     __shared__ __align__(16) float pq[kOriRows * kOS];
-    fill_q(q);
-    __syncthreads();
     const int lane = threadIdx.x;
     const int n = (int)min(*total, (unsigned)cap);
     for (int c = blockIdx.x; c < n; c += gridDim.x) {
@@ -608,7 +611,7 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
         const float kq = -1.44269504088896341f / denom;
         constexpr float lf = 40.0f;  // m = mag * weight * 2^40
         const bool lds = R <= kOriMaxR;
-        for (int i = lane; i < 8 * 36; i += 64) (&hist[0][0])[i] = 0ull;
+        for (int i = lane; i < 4 * 36; i += 64) (&hist[0][0])[i] = 0ull;
         // the samples' pixels (round(x_o + dx), round(y_o + dy)) and their 4-neighbours
         // lie in columns px0 .. px0 + 2R + 4 and rows py0 .. py0 + 2R + 4
         const int px0 = f2i_sat(roundf(xo)) - R - 2, py0 = f2i_sat(roundf(yo)) - R - 2;
@@ -632,7 +635,8 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                     }
                 }
                 *reinterpret_cast<float4*>(&pq[v * kOS + 4 * wd]) =
-                    make_float4(q[u & 255u], q[(u >> 8) & 255u], q[(u >> 16) & 255u], q[u >> 24]);
+                    make_float4(q255x(u & 255u), q255x((u >> 8) & 255u), q255x((u >> 16) & 255u),
+                                q255x(u >> 24));
             }
         }
         __syncthreads();
@@ -652,8 +656,8 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                 gy = pp[kOS] - pp[-kOS];
             } else {
                 const uint8_t* p = g + (size_t)yi * pitch + xi;
-                gx = q[p[1]] - q[p[-1]];
-                gy = q[p[pitch]] - q[p[-pitch]];
+                gx = q255x(p[1]) - q255x(p[-1]);
+                gy = q255x(p[pitch]) - q255x(p[-pitch]);
             }
             const float fdx = (float)dx, fdy = (float)dy;
             const float m = __builtin_amdgcn_sqrtf(__builtin_fmaf(gx, gx, gy * gy)) *
@@ -668,14 +672,14 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
             // floor), the remainder x - hi * 2^32 is exact (Sterbenz), both truncate
             const float hi = floorf(m * 2.3283064365386963e-10f);
             const float lo = __builtin_fmaf(-hi, 4294967296.0f, m);
-            atomicAdd(&hist[lane & 7][bin], ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+            atomicAdd(&hist[lane & 3][bin], ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
         }
         __syncthreads();
         float hv = 0.0f;
         if (lane < 36) {
             unsigned long long sum = 0;
 #pragma unroll
-            for (int k = 0; k < 8; k++) sum += hist[k][lane];
+            for (int k = 0; k < 4; k++) sum += hist[k][lane];
             hv = (float)((double)sum * (1.0 / 1099511627776.0));
         }
         for (int it = 0; it < 2; it++) {
