@@ -193,11 +193,13 @@ __global__ __launch_bounds__(kThreads) void k_extrema(const OctaveDesc* __restri
 }
 
 // ---------------------------------------------------------------------------
-// k_extrema_tile<S>: the same test on 64x16 tiles. All S+3 Gaussian levels of
-// the tile plus a 1-pixel halo are staged once in LDS as q = G/255 floats; lane
-// l owns column x0+l and walks down 4 rows (one wave per 4 rows). Per row it
-// forms the DoG values of every layer at x-1, x, x+1 and their horizontal max /
-// min; a rolling window of three rows gives the 3x3 max / min of each layer.
+// k_extrema_tile<S>: the same test on 64x16 tiles. The S+2 DoG layers of the
+// tile plus a 1-pixel halo are staged once in LDS as f32 (each thread reads one
+// 4-byte word of every Gaussian level and writes q[G_{l+1}] - q[G_l] for its 4
+// pixels); lane l owns column x0+l and walks down 4 rows (one wave per 4 rows).
+// Per row it reads the DoG values of every layer at x-1, x, x+1 and forms their
+// horizontal max / min; a rolling window of three rows gives the 3x3 max / min
+// of each layer.
 // The 26-neighbour maximum of layer s is then max(3x3 of s-1, 3x3 of s+1,
 // row max above and below in s, left and right in s), so "v greater than every
 // neighbour" is v > that maximum (likewise for the minimum) -- the reference's
@@ -215,7 +217,7 @@ __global__ __launch_bounds__(kThreads) void k_extrema_tile(const OctaveDesc* __r
     constexpr int NL = S + 2;  // DoG layers
     constexpr int NG = S + 3;  // Gaussian levels
     constexpr int RH = kTileH + 2, RW = kTileW + 2;
-    __shared__ float Q[NG][RH][RW];
+    __shared__ float D[NL][RH][RW];
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int j = find_by(oct, noct, tile, &OctaveDesc::row_begin);
     const OctaveDesc& O = oct[j];
@@ -224,31 +226,41 @@ __global__ __launch_bounds__(kThreads) void k_extrema_tile(const OctaveDesc* __r
     const int w = O.w, h = O.h, border = P.border;
     const int tiles_x = (w + kTileW - 1) / kTileW;
     const int x0 = (t % tiles_x) * kTileW, y0 = (t / tiles_x) * kTileH;
-    // Stage q = G/255 for columns [x0-1, x0+65) of rows [y0-1, y0+17) (clamped):
-    // 4-byte words starting at x0-4 (x0 is a multiple of 64, rows are 64-byte
-    // aligned); words inside [0, w) are read with one load. q is formed as
-    // (float)((double)v * (1/255)), equal to v / 255.0f for every byte value.
+    // Stage D = q[G_{l+1}] - q[G_l] for columns [x0-1, x0+65) of rows
+    // [y0-1, y0+17) (clamped): 4-byte words starting at x0-4 (x0 is a multiple of
+    // 64, rows are 64-byte aligned); words inside [0, w) are read with one load.
+    // q is formed as (float)((double)v * (1/255)), equal to v / 255.0f for every
+    // byte value, and D is the same f32 subtraction the reference makes.
     constexpr int NW = (kTileW + 8) / 4;  // words covering [x0-4, x0+68)
-    for (int i = threadIdx.x; i < NG * RH * NW; i += kThreads) {
-        const int l = i / (RH * NW);
-        const int rem = i - l * (RH * NW);
-        const int r = rem / NW, k = rem - r * NW;
+    for (int i = threadIdx.x; i < RH * NW; i += kThreads) {
+        const int r = i / NW, k = i - r * NW;
         const int y = min(max(y0 - 1 + r, 0), h - 1);
-        const uint8_t* row = O.levels + (long long)l * O.level_stride + (size_t)y * O.pitch;
         const int xw = x0 - 4 + 4 * k;
-        unsigned u;
-        if (xw >= 0 && xw + 4 <= w) {
-            u = *(const unsigned*)(row + xw);
-        } else {
-            u = 0;
+        const bool whole = xw >= 0 && xw + 4 <= w;
+        unsigned u[NG];
 #pragma unroll
-            for (int b = 0; b < 4; b++) u |= (unsigned)row[min(max(xw + b, 0), w - 1)] << (8 * b);
+        for (int l = 0; l < NG; l++) {
+            const uint8_t* row = O.levels + (long long)l * O.level_stride + (size_t)y * O.pitch;
+            if (whole) {
+                u[l] = *(const unsigned*)(row + xw);
+            } else {
+                u[l] = 0;
+#pragma unroll
+                for (int b = 0; b < 4; b++) u[l] |= (unsigned)row[min(max(xw + b, 0), w - 1)] << (8 * b);
+            }
         }
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             const int c = 4 * k + b - 3;  // column x0-1+c
-            if (c >= 0 && c < RW)
-                Q[l][r][c] = (float)((double)((u >> (8 * b)) & 0xffu) * (1.0 / 255.0));
+            if (c >= 0 && c < RW) {
+                float qlo = (float)((double)(u[0] >> (8 * b) & 0xffu) * (1.0 / 255.0));
+#pragma unroll
+                for (int l = 0; l < NL; l++) {
+                    const float qhi = (float)((double)(u[l + 1] >> (8 * b) & 0xffu) * (1.0 / 255.0));
+                    D[l][r][c] = qhi - qlo;
+                    qlo = qhi;
+                }
+            }
         }
     }
     __syncthreads();
@@ -260,9 +272,9 @@ __global__ __launch_bounds__(kThreads) void k_extrema_tile(const OctaveDesc* __r
     auto load_row = [&](int slot, int r) {
 #pragma unroll
         for (int l = 0; l < NL; l++) {
-            const float a = Q[l + 1][r][lane] - Q[l][r][lane];
-            const float b = Q[l + 1][r][lane + 1] - Q[l][r][lane + 1];
-            const float c = Q[l + 1][r][lane + 2] - Q[l][r][lane + 2];
+            const float a = D[l][r][lane];
+            const float b = D[l][r][lane + 1];
+            const float c = D[l][r][lane + 2];
             dl[slot][l] = a;
             dc[slot][l] = b;
             dr[slot][l] = c;
