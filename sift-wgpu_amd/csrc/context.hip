@@ -662,7 +662,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             vs_rows = std::max(vs_rows, std::min(h, 4 + 2 * r));
         } else if (w >= 192 && R.ntaps <= 64) {
             V.tiles_x = (w + 255) / 256;
-            V.tiles = V.tiles_x * ((h + 31) / 32);
+            V.tiles = V.tiles_x * ((h + 15) / 16);
             vj.push_back(V);
         } else if (narrow_rows * 64 <= 65536) {
             V.tiles_x = (w + 63) / 64;

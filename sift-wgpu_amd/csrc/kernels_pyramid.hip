@@ -18,7 +18,8 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kHTileH = 4;    // one row per wave
 constexpr int kVTileW = 256;  // 64 lanes x 4 px
-constexpr int kVTileH = 32;   // 4 waves x 8 rows
+constexpr int kVRows = 4;     // rows per lane in k_blur_v
+constexpr int kVTileH = 4 * kVRows;  // 4 waves x kVRows rows
 
 template <typename Job>
 __device__ __forceinline__ int find_job(const Job* __restrict__ jobs, int njobs, int tile) {
@@ -160,9 +161,10 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Vertical pass, wide octaves. Tile = 256 columns x 32 rows; lane l owns columns
-// x0+4l..+3 (one u32 load per input row), wave w owns rows y0+8w..+7. A register
-// window of 16 input rows x 4 columns slides over the taps in blocks of 8.
+// Vertical pass, wide octaves. Tile = 256 columns x 16 rows; lane l owns columns
+// x0+4l..+3 (one u32 load per input row), wave w owns rows y0+4w..+3. A register
+// window of 8 input rows x 4 columns slides over the taps in blocks of 4 (about
+// 64 VGPRs, so 8 waves fit per SIMD).
 // Optional DoG epilogue: dog = q[G_s] - q[G_{s-1}], q[v] = v / 255.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void cvt4(unsigned v, float* o) {
@@ -203,67 +205,68 @@ __global__ __launch_bounds__(kThreads) void k_blur_v(const BlurJob* __restrict__
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int w = J.w, h = J.h;
     const int x = tx * kVTileW + lane * 4;
-    const int yb = ty * kVTileH + wave * 8;
+    const int yb = ty * kVTileH + wave * kVRows;
     if (yb >= h || x >= w) return;
     const int r = J.ntaps >> 1;
     const int n = J.ntaps;
     const int pitch = J.src_pitch;
     const uint8_t* col = J.src + x;
     const float* __restrict__ taps = J.taps;
+    constexpr int RV = kVRows;
 
-    float acc[8][4];
+    float acc[RV][4];
 #pragma unroll
-    for (int q = 0; q < 8; q++)
+    for (int q = 0; q < RV; q++)
 #pragma unroll
         for (int c = 0; c < 4; c++) acc[q][c] = 0.0f;
 
-    float lo[8][4];
+    float lo[RV][4];
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
+    for (int i = 0; i < RV; i++) {
         int yy = min(max(yb - r + i, 0), h - 1);
         cvt4(*(const unsigned*)(col + (size_t)yy * pitch), lo[i]);
     }
     int b = 0;
-    for (; b + 8 <= n; b += 8) {
-        float hi[8][4];
+    for (; b + RV <= n; b += RV) {
+        float hi[RV][4];
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            int yy = min(max(yb - r + b + 8 + i, 0), h - 1);
+        for (int i = 0; i < RV; i++) {
+            int yy = min(max(yb - r + b + RV + i, 0), h - 1);
             cvt4(*(const unsigned*)(col + (size_t)yy * pitch), hi[i]);
         }
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (int k = 0; k < RV; k++) {
             const float wk = taps[b + k];
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
+            for (int q = 0; q < RV; q++) {
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
-                    const float p = (q + k < 8) ? lo[q + k][c] : hi[q + k - 8][c];
+                    const float p = (q + k < RV) ? lo[q + k][c] : hi[q + k - RV][c];
                     acc[q][c] = mul_add_exact(acc[q][c], p, wk);
                 }
             }
         }
 #pragma unroll
-        for (int i = 0; i < 8; i++)
+        for (int i = 0; i < RV; i++)
 #pragma unroll
             for (int c = 0; c < 4; c++) lo[i][c] = hi[i][c];
     }
-    // remaining taps (< 8): lo holds rows yb - r + b .. +7
+    // remaining taps (< RV): lo holds rows yb - r + b .. + RV - 1
     for (; b < n; b++) {
         const float wk = taps[b];
 #pragma unroll
-        for (int q = 0; q < 8; q++)
+        for (int q = 0; q < RV; q++)
 #pragma unroll
             for (int c = 0; c < 4; c++) acc[q][c] = mul_add_exact(acc[q][c], lo[q][c], wk);
 #pragma unroll
-        for (int i = 0; i < 7; i++)
+        for (int i = 0; i < RV - 1; i++)
 #pragma unroll
             for (int c = 0; c < 4; c++) lo[i][c] = lo[i + 1][c];
-        int yy = min(max(yb - r + b + 8, 0), h - 1);
-        cvt4(*(const unsigned*)(col + (size_t)yy * pitch), lo[7]);
+        int yy = min(max(yb - r + b + RV, 0), h - 1);
+        cvt4(*(const unsigned*)(col + (size_t)yy * pitch), lo[RV - 1]);
     }
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < RV; q++) {
         const int y = yb + q;
         if (y >= h) break;
         store_row4(J, y, x, acc[q]);
