@@ -72,18 +72,20 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
     const int rowlen = TW + npad + 8;
     const int rowstride = hpad(rowlen) + 1;
 
-    // Stage rows y0..y0+3 (clamped), columns [x0-r, x0-r+rowlen) (clamped) as f32.
-    // Aligned 4-byte words fully inside [0, w) are read with one load; words at
-    // the image edges (and unaligned sources) fall back to clamped bytes.
+    // Wave w stages row y0+w, columns [x0-r, x0-r+rowlen) (clamped) as f32, and
+    // reads only that row, so the waves need no work-group barrier. Aligned
+    // 4-byte words fully inside [0, w) are read with one load; words at the
+    // image edges (and unaligned sources) fall back to clamped bytes.
     const int xs = x0 - r;
     const int ws = xs & ~3;                      // first word start (may be < 0)
     const int nwords = (xs + rowlen - ws + 3) >> 2;
     const bool aligned = (((uintptr_t)J.src | (uintptr_t)J.src_pitch) & 3) == 0;
-    for (int it = threadIdx.x; it < kHTileH * nwords; it += kThreads) {
-        const int rr = it / nwords, k = it - rr * nwords;
-        const int y = min(y0 + rr, h - 1);
-        const uint8_t* src = J.src + (size_t)y * J.src_pitch;
-        float* dstrow = lds + rr * rowstride;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int y = y0 + wave;
+    if (y >= h) return;
+    const uint8_t* src = J.src + (size_t)y * J.src_pitch;
+    float* dstrow = lds + wave * rowstride;
+    for (int k = lane; k < nwords; k += 64) {
         const int xw = ws + 4 * k;
         const int c0 = xw - xs;
         float v[4];
@@ -101,11 +103,10 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
         for (int b = 0; b < 4; b++)
             if (c0 + b >= 0 && c0 + b < rowlen) dstrow[hpad(c0 + b)] = v[b];
     }
-    __syncthreads();
-
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int y = y0 + wave;
-    if (y >= h) return;
+    // the row's LDS writes are complete and visible to the wave before its reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const float* row = lds + wave * rowstride;
     const int base = lane * PX;
     const float* __restrict__ taps = J.taps;
