@@ -1003,6 +1003,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
     __shared__ float rwy[64];
     __shared__ int rpre[65];
     __shared__ int seg_len;
+    __shared__ float2 gcol[kGS], grw[kGS];  // grid columns / rows: texel offset, fraction
     __shared__ float pang[kMaxPeaks], pcos[kMaxPeaks], psin[kMaxPeaks];
     uint8_t* const patch = reinterpret_cast<uint8_t*>(pu);
     unsigned* const hist = pu;
@@ -1062,16 +1063,27 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                     patch[v * kPS + poff + u] = G.g[(size_t)yy * G.pitch + xx];
                 }
             }
+            // the grid's column and row terms: x = x_o + i (and y = y_o + j) are the
+            // same f32 sums for every point of a column (row), so their texel offset
+            // and fraction are formed once
+            if (tid < gside) {
+                const float x = xo + (float)(tid - R - 1);
+                const float xf = floorf(x);
+                gcol[tid] = make_float2(__int_as_float(min(max(f2i_sat(xf) - px0, 0), pside - 2)), x - xf);
+            } else if (tid >= 128 && tid < 128 + gside) {
+                const int gy = tid - 128;
+                const float y = yo + (float)(gy - R - 1);
+                const float yf = floorf(y);
+                const int v0 = min(max(f2i_sat(yf) - py0, 0), pside - 2);
+                grw[gy] = make_float2(__int_as_float(v0 * kPS + poff), y - yf);
+            }
             __syncthreads();
             const float inv_gs = 1.0f / (float)gside;
             for (int i = tid; i < gside * gside; i += kThreads) {
                 const int gy = (int)(((float)i + 0.5f) * inv_gs), gx = i - gy * gside;
-                const float x = xo + (float)(gx - R - 1), y = yo + (float)(gy - R - 1);
-                const float xf = floorf(x), yf = floorf(y);
-                const float dx = x - xf, dy = y - yf;
-                const int u0 = min(max(f2i_sat(xf) - px0, 0), pside - 2);
-                const int v0 = min(max(f2i_sat(yf) - py0, 0), pside - 2);
-                const uint8_t* pp = patch + v0 * kPS + poff + u0;
+                const float2 cxd = gcol[gx], ryd = grw[gy];
+                const float dx = cxd.y, dy = ryd.y;
+                const uint8_t* pp = patch + __float_as_int(ryd.x) + __float_as_int(cxd.x);
                 const float q11 = q[pp[0]], q21 = q[pp[1]];
                 const float q12 = q[pp[kPS]], q22 = q[pp[kPS + 1]];
                 grid[gy * kGS + gx] = q11 * (1.0f - dx) * (1.0f - dy) + q21 * dx * (1.0f - dy) +
