@@ -594,6 +594,7 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
     // work-groups (one wave each) fit on a CU
     __shared__ unsigned long long hist[4][36];
     __shared__ __align__(16) float pq[kOriRows * kOS];
+    __shared__ int ocx[2 * kOriMaxR + 1], ocy[2 * kOriMaxR + 1];
     const int lane = threadIdx.x;
     const int n = (int)min(*total, (unsigned)cap);
     for (int c = blockIdx.x; c < n; c += gridDim.x) {
@@ -650,6 +651,14 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                     make_float4(q255x(u & 255u), q255x((u >> 8) & 255u), q255x((u >> 16) & 255u),
                                 q255x(u >> 24));
             }
+            // per column and row of the window: the sample pixel's offset in the
+            // staged window, or -1 when it is outside [1, w-2] x [1, h-2]
+            if (lane < side) {
+                const int xi = f2i_sat(roundf(xo + (float)(lane - R)));
+                ocx[lane] = (xi >= 1 && xi < w - 1) ? xi - px0a : -1;
+                const int yi = f2i_sat(roundf(yo + (float)(lane - R)));
+                ocy[lane] = (yi >= 1 && yi < h - 1) ? (yi - py0) * kOS : -1;
+            }
         }
         __syncthreads();
         const float inv_side = 1.0f / (float)side;
@@ -658,15 +667,17 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
             if (i >= nsamp) break;
             const int iy = (int)(((float)i + 0.5f) * inv_side);
             const int dy = iy - R, dx = i - iy * side - R;
-            const int yi = f2i_sat(roundf(yo + (float)dy));
-            const int xi = f2i_sat(roundf(xo + (float)dx));
-            if (yi < 1 || yi >= h - 1 || xi < 1 || xi >= w - 1) continue;
             float gx, gy;
             if (lds) {
-                const float* pp = &pq[(yi - py0) * kOS + (xi - px0a)];
+                const int ox = ocx[dx + R], oy = ocy[dy + R];
+                if ((ox | oy) < 0) continue;
+                const float* pp = &pq[oy + ox];
                 gx = pp[1] - pp[-1];
                 gy = pp[kOS] - pp[-kOS];
             } else {
+                const int yi = f2i_sat(roundf(yo + (float)dy));
+                const int xi = f2i_sat(roundf(xo + (float)dx));
+                if (yi < 1 || yi >= h - 1 || xi < 1 || xi >= w - 1) continue;
                 const uint8_t* p = g + (size_t)yi * pitch + xi;
                 gx = q255x(p[1]) - q255x(p[-1]);
                 gy = q255x(p[pitch]) - q255x(p[-pitch]);
