@@ -736,7 +736,7 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
 }
 
 // ---------------------------------------------------------------------------
-// Descriptor helpers: clamped u8 sample, get_gauss_pixel_bilinear, wave sum.
+// Descriptor helpers: clamped u8 sample, get_gauss_pixel_bilinear.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float gauss_px(const uint8_t* g, int pitch, int w, int h, int x, int y,
                                           const float* q) {
@@ -758,12 +758,6 @@ __device__ __forceinline__ float bilinear(const uint8_t* g, int pitch, int w, in
     const float q22 = gauss_px(g, pitch, w, h, x1, y1, q);
     return q11 * (1.0f - dx) * (1.0f - dy) + q21 * dx * (1.0f - dy) + q12 * (1.0f - dx) * dy +
            q22 * dx * dy;
-}
-
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
 }
 
 // Inclusive prefix sum over the 64 lanes of a fully active wave with DPP:

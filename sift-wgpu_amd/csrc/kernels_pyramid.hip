@@ -386,13 +386,30 @@ __global__ __launch_bounds__(kThreads) void k_blur_h_small(const BlurJob* __rest
     J.dst[(size_t)y * J.dst_pitch + x] = (uint8_t)u8_trunc(acc);
 }
 
+// acc + p * tp[k] for k in [k0, k1) in order (one input value, a run of taps):
+// the taps are read 4 at a time once k is a multiple of 4 (tp is 16-byte aligned).
+__device__ __forceinline__ float run_const(float acc, float p, const float* tp, int k0, int k1) {
+    int k = k0;
+    for (; k < k1 && (k & 3); k++) acc = mul_add_exact(acc, p, tp[k]);
+#pragma unroll 2
+    for (; k + 4 <= k1; k += 4) {
+        const float4 t4 = *reinterpret_cast<const float4*>(tp + k);
+        acc = mul_add_exact(acc, p, t4.x);
+        acc = mul_add_exact(acc, p, t4.y);
+        acc = mul_add_exact(acc, p, t4.z);
+        acc = mul_add_exact(acc, p, t4.w);
+    }
+    for (; k < k1; k++) acc = mul_add_exact(acc, p, tp[k]);
+    return acc;
+}
+
 // Vertical: the tile's distinct input rows [max(0, y0-r), min(h, y0+4+r)) are
 // staged as f32 (at most h rows). For output row y, taps k < r-y read row 0 and
 // taps k >= h+r-y read row h-1 (clamp to edge): those runs need no loads and
 // their bounds are the same for the whole wave.
 __global__ __launch_bounds__(kThreads) void k_blur_v_small(const BlurJob* __restrict__ jobs,
                                                            int njobs, int max_taps4) {
-    extern __shared__ float sm[];
+    extern __shared__ __align__(16) float sm[];
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const BlurJob& J = jobs[find_job(jobs, njobs, tile)];
     const int t = tile - J.tile_begin;
@@ -414,17 +431,11 @@ __global__ __launch_bounds__(kThreads) void k_blur_v_small(const BlurJob* __rest
     const int ka = min(max(r - y, 0), n);      // taps reading row 0
     const int kc = max(min(h + r - y, n), ka);  // taps from kc on read row h-1
     float acc = 0.0f;
-    if (ka > 0) {
-        const float p0 = col[(0 - rlo) * kSTileW + lane];
-        for (int k = 0; k < ka; k++) acc = mul_add_exact(acc, p0, tp[k]);
-    }
+    if (ka > 0) acc = run_const(acc, col[(0 - rlo) * kSTileW + lane], tp, 0, ka);
     const float* cp = col + (y - r - rlo) * kSTileW + lane;
 #pragma unroll 4
     for (int k = ka; k < kc; k++) acc = mul_add_exact(acc, cp[k * kSTileW], tp[k]);
-    if (kc < n) {
-        const float pl = col[(h - 1 - rlo) * kSTileW + lane];
-        for (int k = kc; k < n; k++) acc = mul_add_exact(acc, pl, tp[k]);
-    }
+    if (kc < n) acc = run_const(acc, col[(h - 1 - rlo) * kSTileW + lane], tp, kc, n);
     const unsigned v = u8_trunc(acc);
     J.dst[(size_t)y * J.dst_pitch + x] = (uint8_t)v;
     if (J.dog)

@@ -34,7 +34,6 @@ inline int pixel_bytes(uint32_t format) {
 
 struct TapsRef {
     int ntaps = 0, npad = 0;
-    float eps_rel = 0.0f;
     size_t off = 0;      // byte offset in the table store
     bool copy = false;   // blur sigma < 1e-4: the level is a copy of the previous one
 };
