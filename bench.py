@@ -30,7 +30,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=32, help="1080p frames per step")
+    ap.add_argument("--batch", type=int, default=64, help="1080p frames per step")
     ap.add_argument("--latency-iters", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget for the CPU baseline sample (0 disables it)")
@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the served open-loop stream (0 disables it)")
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="requests/s of the stream")
-    ap.add_argument("--traffic", default=os.path.join(REPO, "profiles", "traffic_1080p_b32.txt"),
+    ap.add_argument("--traffic", default=os.path.join(REPO, "profiles", "traffic_1080p_b64.txt"),
                     help="rocprofv3 FETCH/WRITE summary giving HBM bytes per launch")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (for scaling "
