@@ -43,6 +43,14 @@ __device__ __forceinline__ float mul_add_exact(float acc, float a, float b) {
 
 __device__ __forceinline__ float q255(unsigned v) { return __fdiv_rn((float)v, 255.0f); }
 
+// The tap tables are written by host copies before any launch and only read
+// by kernels: reading them through the constant address space lets the
+// wave-uniform tap loads go through scalar loads instead of vector loads.
+typedef __attribute__((address_space(4))) const float ctap_t;
+__device__ __forceinline__ const ctap_t* as_const_taps(const float* p) {
+    return (const ctap_t*)(p);
+}
+
 // ---------------------------------------------------------------------------
 // Horizontal pass. Tile = (64*PX) columns x 4 rows; wave w computes row y0+w;
 // lane l computes PX consecutive outputs. The row segment [x0-r, x0+64*PX+pad)
@@ -109,7 +117,7 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const float* row = lds + wave * rowstride;
     const int base = lane * PX;
-    const float* __restrict__ taps = J.taps;
+    const ctap_t* taps = as_const_taps(J.taps);
     // Taps in blocks of 4 (taps beyond ntaps are zero, <= ntaps_pad). When one tap
     // is left over (ntaps = 4k + 1), it is applied after the blocks from the window
     // registers, which then hold row[base + 4k ..], instead of a block of three
@@ -212,7 +220,7 @@ __global__ __launch_bounds__(kThreads) void k_blur_v(const BlurJob* __restrict__
     const int n = J.ntaps;
     const int pitch = J.src_pitch;
     const uint8_t* col = J.src + x;
-    const float* __restrict__ taps = J.taps;
+    const ctap_t* taps = as_const_taps(J.taps);
     constexpr int RV = kVRows;
 
     float acc[RV][4];
@@ -306,7 +314,7 @@ __global__ __launch_bounds__(kThreads) void k_blur_v_narrow(const BlurJob* __res
     const int x = x0 + lane;
     const int yb = y0 + wave * 4;
     if (x >= w || yb >= h) return;
-    const float* __restrict__ taps = J.taps;
+    const ctap_t* taps = as_const_taps(J.taps);
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     float cur[4];
 #pragma unroll
