@@ -566,7 +566,7 @@ __global__ __launch_bounds__(kThreads) void k_resize_v(const ResizeJob* __restri
     const int x = ((t % J.tiles_x) * kThreads + threadIdx.x) * 4;
     if (x >= J.in_w) return;
     const int l = J.left[oy], n = J.count[oy];
-    const float* wr = J.weights + (size_t)oy * J.maxtaps;
+    const ctap_t* wr = as_const_taps(J.weights) + (size_t)oy * J.maxtaps;  // one row for the wave
     const uint8_t* src = J.src_u8 + (size_t)l * J.src_pitch + x;
     float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     int i = 0;
