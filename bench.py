@@ -164,7 +164,7 @@ def served(seconds, rate, seed=7):
 
 def frames(n):
     import imgen
-    w, h, b, r, noise, seed, octv, ct = imgen.CONFIGS["c2"]
+    w, h, b, r, noise, seed, octv, ct = imgen.CONFIGS["1920x1080"]
     return [imgen.synth_image(w, h, b, r, noise, seed + 1000 * i) for i in range(n)], octv, ct
 
 

@@ -1,9 +1,9 @@
 """Seeded synthetic grey-scale images (ctypes binding of csrc/imgen.c).
 
 Workloads (the shapes, counts and seeds the tests and the benchmark use):
-  c1    512x512,   200 blobs,  50 rects, noise 0.01, seed 1, 4 octaves
-  c2    1920x1080, 800 blobs, 200 rects, noise 0.01, seed 2, 5 octaves
-  c3    3840x2160, 3000 blobs, 800 rects, noise 0.02, seed 3, 6 octaves, contrast 0.03
+  512x512     200 blobs,  50 rects, noise 0.01, seed 1, 4 octaves
+  1920x1080   800 blobs, 200 rects, noise 0.01, seed 2, 5 octaves
+  3840x2160   3000 blobs, 800 rects, noise 0.02, seed 3, 6 octaves, contrast 0.03
   batch 64 images, sizes uniform over 640x480/1280x720/1920x1080/2048x1536,
         image i seed 1000+i, octaves floor(log2(min(w,h)))-3
 """
@@ -43,9 +43,9 @@ def synth_image(w, h, n_blobs, n_rects, noise_sigma, seed):
 
 # name -> (w, h, blobs, rects, noise, seed, num_octaves, contrast_threshold)
 CONFIGS = {
-    "c1": (512, 512, 200, 50, 0.01, 1, 4, 0.04),
-    "c2": (1920, 1080, 800, 200, 0.01, 2, 5, 0.04),
-    "c3": (3840, 2160, 3000, 800, 0.02, 3, 6, 0.03),
+    "512x512": (512, 512, 200, 50, 0.01, 1, 4, 0.04),
+    "1920x1080": (1920, 1080, 800, 200, 0.01, 2, 5, 0.04),
+    "3840x2160": (3840, 2160, 3000, 800, 0.02, 3, 6, 0.03),
 }
 
 BATCH_SHAPES = [(640, 480), (1280, 720), (1920, 1080), (2048, 1536)]

@@ -1,5 +1,5 @@
 """Writes the committed golden fixtures for the CPU oracle:
-  images.json        SHA-256 of the seeded workload images (c1, c2, c3)
+  images.json        SHA-256 of the seeded workload images (512x512, 1920x1080, 3840x2160)
   oracle_small.json  pyramid / candidate / refined hashes and counts for small
                      seeded images, plus oracle_small_*.npz with the final
                      keypoints and descriptors.
@@ -28,11 +28,11 @@ CASES = [
 
 def main():
     imgs = {}
-    for name in ("c1", "c2", "c3"):
+    for name in ("512x512", "1920x1080", "3840x2160"):
         img, _, _ = imgen.config_image(name)
         imgs[name] = hashlib.sha256(img.tobytes()).hexdigest()
     with open(os.path.join(HERE, "images.json"), "w") as f:
-        json.dump({"configs": imgs}, f, indent=1)
+        json.dump({"workloads": imgs}, f, indent=1)
     out = []
     for i, c in enumerate(CASES):
         img = imgen.generic_image(c["w"], c["h"], seed=c["seed"])

@@ -241,7 +241,7 @@ def test_small_gradient_image():
 
 def test_determinism_and_properties():
     # tests/integration_test.rs:64-164
-    img, octv, ct = imgen.config_image("c1")
+    img, octv, ct = imgen.config_image("512x512")
     p = oracle.params(1.6, octv, 3, 0.5, ct, 10.0)
     k1, d1 = oracle.detect_and_compute(p, img, nthreads=4)
     k2, d2 = oracle.detect_and_compute(p, img, nthreads=1)
@@ -254,7 +254,7 @@ def test_determinism_and_properties():
 
 def test_high_contrast_threshold():
     # tests/integration_test.rs:228-281
-    img, octv, _ = imgen.config_image("c1")
+    img, octv, _ = imgen.config_image("512x512")
     kd, _ = oracle.detect_and_compute(oracle.params(1.6, octv, 3, 0.5, 0.04, 10.0), img)
     kh, dh = oracle.detect_and_compute(oracle.params(1.6, octv, 3, 0.5, 1.0, 10.0), img)
     assert len(kh) < len(kd) // 2 or len(kh) == 0
@@ -266,7 +266,7 @@ def test_high_contrast_threshold():
 def test_generator_checksums():
     with open(os.path.join(GOLDEN, "images.json")) as f:
         ref = json.load(f)
-    for name, sha in ref["configs"].items():
+    for name, sha in ref["workloads"].items():
         img, _, _ = imgen.config_image(name)
         assert hashlib.sha256(img.tobytes()).hexdigest() == sha, name
 

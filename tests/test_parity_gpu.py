@@ -24,9 +24,9 @@ def _sift(octaves=4, contrast=0.04, S=3, edge=10.0, sigma=1.6, blur=0.5):
 
 
 CASES = {
-    "c1": lambda: imgen.config_image("c1"),
-    "c2": lambda: imgen.config_image("c2"),
-    "c3": lambda: imgen.config_image("c3"),  # 3840x2160, 6 octaves, contrast 0.03
+    "512x512": lambda: imgen.config_image("512x512"),
+    "1920x1080": lambda: imgen.config_image("1920x1080"),
+    "3840x2160": lambda: imgen.config_image("3840x2160"),  # 3840x2160, 6 octaves, contrast 0.03
     "odd_97x61": lambda: (imgen.generic_image(97, 61, seed=9), 3, 0.04),
     "wide_333x45": lambda: (imgen.generic_image(333, 45, seed=10), 4, 0.04),
 }
@@ -43,7 +43,7 @@ def test_pyramid_and_dog_bit_exact(case):
         assert np.array_equal(d_hip[o], d_ref[o]), f"dog octave {o}"
 
 
-@pytest.mark.parametrize("case", ["c1", "c2", "c3", "odd_97x61"])
+@pytest.mark.parametrize("case", ["512x512", "1920x1080", "3840x2160", "odd_97x61"])
 def test_candidates_and_refined_bit_exact(case):
     img, octv, ct = CASES[case]()
     s, p = _sift(octv, ct)
@@ -65,7 +65,7 @@ def _parity(img, s, p):
     return kps, desc, okps, m
 
 
-@pytest.mark.parametrize("case", ["c1", "c2", "c3", "odd_97x61", "wide_333x45"])
+@pytest.mark.parametrize("case", ["512x512", "1920x1080", "3840x2160", "odd_97x61", "wide_333x45"])
 def test_keypoints_and_descriptors_match_oracle(case):
     img, octv, ct = CASES[case]()
     s, p = _sift(octv, ct)
@@ -120,7 +120,7 @@ def test_tiny_and_degenerate_images(shape):
 
 def test_determinism():
     # tests/integration_test.rs:106-164: bitwise-equal repeated runs
-    img, octv, ct = imgen.config_image("c1")
+    img, octv, ct = imgen.config_image("512x512")
     s, _ = _sift(octv, ct)
     k1, d1 = s.detect_and_compute(img)
     k2, d2 = s.detect_and_compute(img)
@@ -143,7 +143,7 @@ def test_batch_equals_alone():
 
 def test_high_contrast_threshold():
     # tests/integration_test.rs:228-281
-    img, octv, _ = imgen.config_image("c1")
+    img, octv, _ = imgen.config_image("512x512")
     kd, _ = sift_hip.Sift(1.6, octv, 3, 0.5, 0.04, 10.0).detect_and_compute(img)
     kh, dh = sift_hip.Sift(1.6, octv, 3, 0.5, 1.0, 10.0).detect_and_compute(img)
     assert len(kd) > 0
