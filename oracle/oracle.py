@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/libsift_oracle.so).
 
 TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline,
+cpu_baseline leg of bench.py, always as the parity reference or the timed CPU
+baseline,
 never as the product path. The oracle restates src/sift.rs of the reference
 (see sift_oracle.c for the per-function citations).
 """

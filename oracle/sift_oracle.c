@@ -1,7 +1,8 @@
 /*
  * sift_oracle.c -- TEST INFRASTRUCTURE. A scalar C restatement of the reference
  * CPU SIFT path (stevel705/sift-wgpu, src/sift.rs:1200-1240), used as the parity
- * checker and, timed, as the CPU baseline. Nothing in the product links it.
+ * reference of the tests and, timed, as the CPU baseline. Nothing in the
+ * product links it.
  *
  * Build: oracle/Makefile, gcc -O2 -ffp-contract=off (no FMA contraction, SSE
  * float arithmetic, FLT_EVAL_METHOD == 0) so that every f32 operation rounds

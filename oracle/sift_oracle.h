@@ -1,6 +1,7 @@
 /*
  * sift_oracle.h -- CPU restatement of the reference SIFT path, used ONLY as
- * test infrastructure (the parity checker and the CPU baseline timing).
+ * test infrastructure (the reference the parity tests compare against, and
+ * the CPU baseline timing).
  *
  * The product (libsift_hip.so) never links, loads or calls anything here.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.

@@ -1,5 +1,5 @@
 """Pins the CPU oracle (oracle/sift_oracle.c) before it is trusted as the parity
-checker: the reference's own known answers, the reference's integration-test
+reference: the reference's own known answers, the reference's integration-test
 invariants, closed forms, an independent library (PIL) and brute force."""
 import hashlib
 import json
