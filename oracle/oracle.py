@@ -2,9 +2,8 @@
 
 TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, always as the parity reference or the timed CPU
-baseline,
-never as the product path. The oracle restates src/sift.rs of the reference
-(see sift_oracle.c for the per-function citations).
+baseline, never as the product path. The oracle restates src/sift.rs of the
+reference (see sift_oracle.c for the per-function citations).
 """
 import ctypes
 import os
