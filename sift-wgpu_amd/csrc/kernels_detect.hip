@@ -976,7 +976,9 @@ __device__ __forceinline__ void desc_finish(const unsigned* hist, float fix, int
 // window, with its clamp-to-edge rule -- is staged in LDS (aligned 32-bit
 // copies when the patch is inside the level, clamped byte loads at the
 // borders), and the bilinear samples on the shifted grid (x_o + i, y_o + j),
-// |i|,|j| <= R+1, are formed once from it. A window sample (px, py) takes its
+// |i|,|j| <= R+1, are formed once from it, in texel units (0..255): gradients
+// are differences of grid values, so their angle is the same and their
+// magnitude is scaled by 1/255 with the Gaussian weight. A window sample (px, py) takes its
 // gradient from the four grid neighbours (the reference's bilinear(x_s +- 1, y_s)
 // and bilinear(x_s, y_s +- 1)).
 // Per peak, the samples of each row whose rotated bin coordinates can fall in
@@ -998,8 +1000,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
     const float* __restrict__ angles, const unsigned* __restrict__ koff, int kcap,
     Keypoint* __restrict__ kps_out, float* __restrict__ desc_out, unsigned* __restrict__ wide_count,
     int* __restrict__ wide_list) {
-    static_assert(kHistCopies * kHistWords * 4 >= kPatchRows * kPS, "patch shares the histograms");
-    __shared__ float q[256];
+    static_assert(kHistCopies * kHistWords * 4 >= kPatchRows * kPS + 4, "patch shares the histograms");
     // the u8 patch is dead once the grid is built; the histograms live after that
     __shared__ __align__(16) unsigned pu[kHistCopies * kHistWords];
     __shared__ float grid[kGS * kGS];
@@ -1012,8 +1013,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
     __shared__ float pang[kMaxPeaks], pcos[kMaxPeaks], psin[kMaxPeaks];
     uint8_t* const patch = reinterpret_cast<uint8_t*>(pu);
     unsigned* const hist = pu;
-    fill_q(q);
-    __syncthreads();
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int n = (int)min(*total, (unsigned)cap);
     for (int c = blockIdx.x; c < n; c += gridDim.x) {
@@ -1088,11 +1087,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                 const int gy = (int)(((float)i + 0.5f) * inv_gs), gx = i - gy * gside;
                 const float2 cxd = gcol[gx], ryd = grw[gy];
                 const float dx = cxd.y, dy = ryd.y;
-                const uint8_t* pp = patch + __float_as_int(ryd.x) + __float_as_int(cxd.x);
-                const float q11 = q[pp[0]], q21 = q[pp[1]];
-                const float q12 = q[pp[kPS]], q22 = q[pp[kPS + 1]];
-                grid[gy * kGS + gx] = q11 * (1.0f - dx) * (1.0f - dy) + q21 * dx * (1.0f - dy) +
-                                      q12 * (1.0f - dx) * dy + q22 * dx * dy;
+                // texels (a, a+1) of two rows from aligned words: a byte-aligned
+                // 16-bit LDS read would be an unaligned access
+                const unsigned a = (unsigned)(__float_as_int(ryd.x) + __float_as_int(cxd.x));
+                const unsigned* wp = pu + (a >> 2);
+                const unsigned sh8 = a & 3u;
+                const unsigned r0 = __builtin_amdgcn_alignbyte(wp[1], wp[0], sh8);
+                const unsigned r1 = __builtin_amdgcn_alignbyte(wp[kPS / 4 + 1], wp[kPS / 4], sh8);
+                const float p11 = (float)(r0 & 255u), p21 = (float)((r0 >> 8) & 255u);
+                const float p12 = (float)(r1 & 255u), p22 = (float)((r1 >> 8) & 255u);
+                const float top = __builtin_fmaf(p21 - p11, dx, p11);
+                const float bot = __builtin_fmaf(p22 - p12, dx, p12);
+                grid[gy * kGS + gx] = __builtin_fmaf(bot - top, dy, top);
             }
             // Gaussian weight exp(-(px^2+py^2)/denom) as exp(-px^2/denom) * exp(-py^2/denom)
             for (int i = tid; i < side; i += kThreads) exs[i] = expf(-(float)((i - R) * (i - R)) / denom);
@@ -1126,7 +1132,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                     cnt = max(b - a + 1, 0);
                     rrec[tid] = make_float4(__int_as_float(lo), __int_as_float(lo + cnt - 1),
                                             sa * py * inv_bw, ca * py * inv_bw);
-                    rwy[tid] = exs[cy] * kDescFix;
+                    rwy[tid] = exs[cy] * (kDescFix / 255.0f);
                 }
                 // Each row is cut into segments of at most `per` consecutive samples,
                 // one segment per thread: per is the smallest value >= T / 256 that
