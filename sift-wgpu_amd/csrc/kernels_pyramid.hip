@@ -55,14 +55,76 @@ __device__ __forceinline__ const ctap_t* as_const_taps(const float* p) {
 // Horizontal pass. Tile = (64*PX) columns x 4 rows; wave w computes row y0+w;
 // lane l computes PX consecutive outputs. The row segment [x0-r, x0+64*PX+pad)
 // is staged in LDS as f32 with clamp-to-edge indexing (the reference's edge
-// rule), with one pad word after every 32 (element c at c + c/32) so that the
-// lanes' reads, 8 elements apart, fall in distinct banks. A register window of
-// PX+4 inputs slides over the taps in blocks of 4; taps are zero-padded to a
-// multiple of 4 (p * 0 added to a non-negative sum leaves it unchanged), except
-// that a single leftover tap is applied on its own, and are wave-uniform
-// (scalar registers).
+// rule), with pad words so that the lanes' reads, PX elements apart, fall in
+// distinct banks: PX = 8 puts one pad word after every 8 elements (element c at
+// c + c/8, lane stride 9 words), which also makes element base + k of every
+// lane sit at the same offset k + k/8 from the lane's first element; PX = 2 puts
+// one after every 32 (c + c/32).
+// For PX = 8 and the tap counts of the default scale schedule (7 to 15 taps) the
+// taps are fully unrolled: the PX + n - 1 window inputs are read once and every
+// tap is a scalar register. Otherwise a register window of PX+4 inputs slides
+// over the taps in blocks of 4; taps are zero-padded to a multiple of 4 (p * 0
+// added to a non-negative sum leaves it unchanged), except that a single
+// leftover tap is applied on its own, and are wave-uniform (scalar registers).
 // ---------------------------------------------------------------------------
-__host__ __device__ __forceinline__ int hpad(int c) { return c + (c >> 5); }
+template <int PX>
+__host__ __device__ __forceinline__ int hpad(int c) {
+    return PX == 8 ? c + (c >> 3) : c + (c >> 5);
+}
+
+// n taps over the window row[b8 + k + k/8], k < PX + n - 1 (b8 = hpad<8> of the
+// lane's first element), in tap order per output; the first product is the
+// first partial sum (0 + p * w0 is p * w0: p * w0 is never -0).
+template <int PX, int NT>
+__device__ __forceinline__ void hconv_fixed(const float* __restrict__ p, const ctap_t* taps,
+                                            float (&acc)[PX]) {
+    constexpr int NW = PX + NT - 1;
+    float win[NW];
+#pragma unroll
+    for (int k = 0; k < NW; k++) win[k] = p[k + (k >> 3)];
+    float wk[NT];
+#pragma unroll
+    for (int k = 0; k < NT; k++) wk[k] = taps[k];
+#pragma unroll
+    for (int q = 0; q < PX; q++) acc[q] = __fmul_rn(win[q], wk[0]);
+#pragma unroll
+    for (int k = 1; k < NT; k++)
+#pragma unroll
+        for (int q = 0; q < PX; q++) acc[q] = mul_add_exact(acc[q], win[q + k], wk[k]);
+}
+
+// Any tap count: taps in blocks of 4 (taps beyond ntaps are zero, <= ntaps_pad).
+// When one tap is left over (ntaps = 4k + 1), it is applied after the blocks from
+// the window registers, which then hold row[base + 4k ..], instead of a block of
+// three zero taps.
+template <int PX>
+__device__ __forceinline__ void hconv_blocks(const float* __restrict__ row, int base,
+                                             const ctap_t* taps, int ntaps, float (&acc)[PX]) {
+    const bool one_left = (ntaps & 3) == 1;
+    const int nb = one_left ? (ntaps & ~3) : ((ntaps + 3) & ~3);
+#pragma unroll
+    for (int q = 0; q < PX; q++) acc[q] = 0.0f;
+    float win[PX + 4];
+#pragma unroll
+    for (int k = 0; k < PX; k++) win[k] = row[hpad<PX>(base + k)];
+    for (int b = 0; b < nb; b += 4) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) win[PX + k] = row[hpad<PX>(base + PX + b + k)];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float wk = taps[b + k];
+#pragma unroll
+            for (int q = 0; q < PX; q++) acc[q] = mul_add_exact(acc[q], win[q + k], wk);
+        }
+#pragma unroll
+        for (int k = 0; k < PX; k++) win[k] = win[k + 4];
+    }
+    if (one_left) {
+        const float wk = taps[nb];
+#pragma unroll
+        for (int q = 0; q < PX; q++) acc[q] = mul_add_exact(acc[q], win[q], wk);
+    }
+}
 
 template <int PX>
 __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__ jobs, int njobs) {
@@ -78,7 +140,7 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
     const int r = J.ntaps >> 1;
     const int npad = J.ntaps_pad;
     const int rowlen = TW + npad + 8;
-    const int rowstride = hpad(rowlen) + 1;
+    const int rowstride = hpad<PX>(rowlen) + 1;
 
     // Wave w stages row y0+w, columns [x0-r, x0-r+rowlen) (clamped) as f32, and
     // reads only that row, so the waves need no work-group barrier. Aligned
@@ -109,7 +171,7 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
         }
 #pragma unroll
         for (int b = 0; b < 4; b++)
-            if (c0 + b >= 0 && c0 + b < rowlen) dstrow[hpad(c0 + b)] = v[b];
+            if (c0 + b >= 0 && c0 + b < rowlen) dstrow[hpad<PX>(c0 + b)] = v[b];
     }
     // the row's LDS writes are complete and visible to the wave before its reads
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -118,36 +180,21 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
     const float* row = lds + wave * rowstride;
     const int base = lane * PX;
     const ctap_t* taps = as_const_taps(J.taps);
-    // Taps in blocks of 4 (taps beyond ntaps are zero, <= ntaps_pad). When one tap
-    // is left over (ntaps = 4k + 1), it is applied after the blocks from the window
-    // registers, which then hold row[base + 4k ..], instead of a block of three
-    // zero taps.
-    const bool one_left = (J.ntaps & 3) == 1;
-    const int nb = one_left ? (J.ntaps & ~3) : ((J.ntaps + 3) & ~3);
-
     float acc[PX];
-#pragma unroll
-    for (int q = 0; q < PX; q++) acc[q] = 0.0f;
-    float win[PX + 4];
-#pragma unroll
-    for (int k = 0; k < PX; k++) win[k] = row[hpad(base + k)];
-    for (int b = 0; b < nb; b += 4) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) win[PX + k] = row[hpad(base + PX + b + k)];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const float wk = taps[b + k];
-#pragma unroll
-            for (int q = 0; q < PX; q++) acc[q] = mul_add_exact(acc[q], win[q + k], wk);
+    bool fixed = false;
+    if constexpr (PX == 8) {
+        const float* p = row + hpad<8>(base);
+        fixed = true;
+        switch (J.ntaps) {
+            case 7: hconv_fixed<8, 7>(p, taps, acc); break;
+            case 9: hconv_fixed<8, 9>(p, taps, acc); break;
+            case 11: hconv_fixed<8, 11>(p, taps, acc); break;
+            case 13: hconv_fixed<8, 13>(p, taps, acc); break;
+            case 15: hconv_fixed<8, 15>(p, taps, acc); break;
+            default: fixed = false;
         }
-#pragma unroll
-        for (int k = 0; k < PX; k++) win[k] = win[k + 4];
     }
-    if (one_left) {
-        const float wk = taps[nb];
-#pragma unroll
-        for (int q = 0; q < PX; q++) acc[q] = mul_add_exact(acc[q], win[q], wk);
-    }
+    if (!fixed) hconv_blocks<PX>(row, base, taps, J.ntaps, acc);
 
     const int x = x0 + base;
     if (x >= w) return;
@@ -171,9 +218,10 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
 
 // ---------------------------------------------------------------------------
 // Vertical pass, wide octaves. Tile = 256 columns x 16 rows; lane l owns columns
-// x0+4l..+3 (one u32 load per input row), wave w owns rows y0+4w..+3. A register
-// window of 8 input rows x 4 columns slides over the taps in blocks of 4 (about
-// 64 VGPRs, so 8 waves fit per SIMD).
+// x0+4l..+3 (one u32 load per input row), wave w owns rows y0+4w..+3. For the
+// tap counts of the default scale schedule (7 to 15) the taps are unrolled and
+// each input row is loaded and converted once; otherwise a register window of
+// 8 input rows x 4 columns slides over the taps in blocks of 4.
 // Optional DoG epilogue: dog = q[G_s] - q[G_{s-1}], q[v] = v / 255.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void cvt4(unsigned v, float* o) {
@@ -205,25 +253,11 @@ __device__ __forceinline__ void store_row4(const BlurJob& J, int y, int x, const
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_blur_v(const BlurJob* __restrict__ jobs, int njobs) {
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    const int j = find_job(jobs, njobs, tile);
-    const BlurJob& J = jobs[j];
-    const int t = tile - J.tile_begin;
-    const int tx = t % J.tiles_x, ty = t / J.tiles_x;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int w = J.w, h = J.h;
-    const int x = tx * kVTileW + lane * 4;
-    const int yb = ty * kVTileH + wave * kVRows;
-    if (yb >= h || x >= w) return;
-    const int r = J.ntaps >> 1;
-    const int n = J.ntaps;
-    const int pitch = J.src_pitch;
-    const uint8_t* col = J.src + x;
-    const ctap_t* taps = as_const_taps(J.taps);
-    constexpr int RV = kVRows;
-
-    float acc[RV][4];
+// Any tap count: a register window of RV input rows x 4 columns slides over the
+// taps in blocks of RV; the remaining taps (< RV) are applied one at a time.
+template <int RV>
+__device__ __forceinline__ void vconv_blocks(const uint8_t* col, int pitch, int y0, int h, int n,
+                                             const ctap_t* taps, float (&acc)[RV][4]) {
 #pragma unroll
     for (int q = 0; q < RV; q++)
 #pragma unroll
@@ -232,7 +266,7 @@ __global__ __launch_bounds__(kThreads) void k_blur_v(const BlurJob* __restrict__
     float lo[RV][4];
 #pragma unroll
     for (int i = 0; i < RV; i++) {
-        int yy = min(max(yb - r + i, 0), h - 1);
+        int yy = min(max(y0 + i, 0), h - 1);
         cvt4(*(const unsigned*)(col + (size_t)yy * pitch), lo[i]);
     }
     int b = 0;
@@ -240,7 +274,7 @@ __global__ __launch_bounds__(kThreads) void k_blur_v(const BlurJob* __restrict__
         float hi[RV][4];
 #pragma unroll
         for (int i = 0; i < RV; i++) {
-            int yy = min(max(yb - r + b + RV + i, 0), h - 1);
+            int yy = min(max(y0 + b + RV + i, 0), h - 1);
             cvt4(*(const unsigned*)(col + (size_t)yy * pitch), hi[i]);
         }
 #pragma unroll
@@ -260,7 +294,7 @@ __global__ __launch_bounds__(kThreads) void k_blur_v(const BlurJob* __restrict__
 #pragma unroll
             for (int c = 0; c < 4; c++) lo[i][c] = hi[i][c];
     }
-    // remaining taps (< RV): lo holds rows yb - r + b .. + RV - 1
+    // remaining taps (< RV): lo holds rows y0 + b .. + RV - 1
     for (; b < n; b++) {
         const float wk = taps[b];
 #pragma unroll
@@ -271,9 +305,69 @@ __global__ __launch_bounds__(kThreads) void k_blur_v(const BlurJob* __restrict__
         for (int i = 0; i < RV - 1; i++)
 #pragma unroll
             for (int c = 0; c < 4; c++) lo[i][c] = lo[i + 1][c];
-        int yy = min(max(yb - r + b + RV, 0), h - 1);
+        int yy = min(max(y0 + b + RV, 0), h - 1);
         cvt4(*(const unsigned*)(col + (size_t)yy * pitch), lo[RV - 1]);
     }
+}
+
+// NT taps, fully unrolled: the RV + NT - 1 input rows y0.. (clamped; y0 is
+// wave-uniform, so each row is one scalar base plus the lane's column offset)
+// are streamed once in increasing order, and input row i adds tap i - q to
+// output row q -- per output, the taps in tap order. The first product is the
+// first partial sum (p * w0 is never -0).
+typedef __attribute__((address_space(1))) const uint8_t gbyte_t;
+
+template <int RV, int NT>
+__device__ __forceinline__ void vconv_fixed(const uint8_t* src, unsigned x, int pitch, int y0,
+                                            int h, const ctap_t* taps, float (&acc)[RV][4]) {
+    float wk[NT];
+#pragma unroll
+    for (int k = 0; k < NT; k++) wk[k] = taps[k];
+#pragma unroll
+    for (int i = 0; i < RV + NT - 1; i++) {
+        const int yy = min(max(y0 + i, 0), h - 1);
+        gbyte_t* rowp = (gbyte_t*)src + (size_t)yy * pitch;
+        float p[4];
+        cvt4(*(__attribute__((address_space(1))) const unsigned*)(rowp + x), p);
+#pragma unroll
+        for (int q = 0; q < RV; q++) {
+            const int k = i - q;
+            if (k < 0 || k >= NT) continue;
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                acc[q][c] = k == 0 ? __fmul_rn(p[c], wk[0]) : mul_add_exact(acc[q][c], p[c], wk[k]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_blur_v(const BlurJob* __restrict__ jobs, int njobs) {
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int j = find_job(jobs, njobs, tile);
+    const BlurJob& J = jobs[j];
+    const int t = tile - J.tile_begin;
+    const int tx = t % J.tiles_x, ty = t / J.tiles_x;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int w = J.w, h = J.h;
+    const int x = tx * kVTileW + lane * 4;
+    const int yb = ty * kVTileH + wave * kVRows;
+    if (yb >= h || x >= w) return;
+    const int r = J.ntaps >> 1;
+    const int n = J.ntaps;
+    const int pitch = J.src_pitch;
+    const ctap_t* taps = as_const_taps(J.taps);
+    constexpr int RV = kVRows;
+
+    float acc[RV][4];
+    bool fixed = true;
+    switch (n) {
+        case 7: vconv_fixed<RV, 7>(J.src, x, pitch, yb - r, h, taps, acc); break;
+        case 9: vconv_fixed<RV, 9>(J.src, x, pitch, yb - r, h, taps, acc); break;
+        case 11: vconv_fixed<RV, 11>(J.src, x, pitch, yb - r, h, taps, acc); break;
+        case 13: vconv_fixed<RV, 13>(J.src, x, pitch, yb - r, h, taps, acc); break;
+        case 15: vconv_fixed<RV, 15>(J.src, x, pitch, yb - r, h, taps, acc); break;
+        default: fixed = false;
+    }
+    if (!fixed) vconv_blocks<RV>(J.src + x, pitch, yb - r, h, n, taps, acc);
 #pragma unroll
     for (int q = 0; q < RV; q++) {
         const int y = yb + q;
@@ -653,7 +747,8 @@ void launch_blur_h(const BlurJob* d_jobs, int njobs, int total_tiles, int max_pa
                    hipStream_t s) {
     if (total_tiles <= 0) return;
     const int rowlen = 64 * px + max_pad + 8;
-    const size_t lds = (size_t)kHTileH * (hpad(rowlen) + 1) * sizeof(float);
+    const size_t lds =
+        (size_t)kHTileH * ((px == 8 ? hpad<8>(rowlen) : hpad<2>(rowlen)) + 1) * sizeof(float);
     if (px == 8)
         hipLaunchKernelGGL(k_blur_h<8>, dim3(total_tiles), dim3(kThreads), lds, s, d_jobs, njobs);
     else
