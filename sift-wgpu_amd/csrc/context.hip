@@ -515,7 +515,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             d.bits_begin = nbits;
             d.nwords = (s.w + 63) / 64;
             d.row_begin = tile_ext ? total_tiles : total_rows;
-            total_tiles += ((s.w + 63) / 64) * ((s.h + 15) / 16);
+            total_tiles += ((s.w + 63) / 64) * ((s.h + kExtStripRows - 1) / kExtStripRows);
             nslots += S * s.h;
             nbits += (long long)S * s.h * d.nwords;
             total_rows += s.h;

@@ -193,13 +193,15 @@ __global__ __launch_bounds__(kThreads) void k_extrema(const OctaveDesc* __restri
 }
 
 // ---------------------------------------------------------------------------
-// k_extrema_tile<S>: the same test on 64x16 tiles. The S+2 DoG layers of the
-// tile plus a 1-pixel halo are staged once in LDS as f32 (each thread reads one
-// 4-byte word of every Gaussian level and writes q[G_{l+1}] - q[G_l] for its 4
-// pixels); lane l owns column x0+l and walks down 4 rows (one wave per 4 rows).
-// Per row it reads the DoG values of every layer at x-1, x, x+1 and forms their
-// horizontal max / min; a rolling window of three rows gives the 3x3 max / min
-// of each layer.
+// k_extrema_tile<S>: the same test on strips of 64 columns x kExtStripRows rows,
+// taken as bands of 16 rows. The S+2 DoG layers of a band plus a 1-pixel halo are
+// staged in LDS as f32 (each thread holds one 8-byte word of every Gaussian level
+// and writes q[G_{l+1}] - q[G_l] for its 8 pixels); the words of the next band
+// are loaded into registers before the current band is tested, so their latency
+// overlaps the test. In a band, lane l owns column x0+l and walks down 4 rows (one
+// wave per 4 rows). Per row it reads the DoG values of every layer at x-1, x, x+1
+// and forms their horizontal max / min; a rolling window of three rows gives the
+// 3x3 max / min of each layer.
 // The 26-neighbour maximum of layer s is then max(3x3 of s-1, 3x3 of s+1,
 // row max above and below in s, left and right in s), so "v greater than every
 // neighbour" is v > that maximum (likewise for the minimum) -- the reference's
@@ -207,17 +209,79 @@ __global__ __launch_bounds__(kThreads) void k_extrema(const OctaveDesc* __restri
 // bitmask word; the row-slot counts are integer atomics (order-independent).
 // ---------------------------------------------------------------------------
 constexpr int kTileW = 64, kTileH = 16;
+static_assert(kExtStripRows % kTileH == 0, "strips are whole bands");
+constexpr int kExtNW = (kTileW + 16) / 8;         // 8-byte words covering [x0-8, x0+72)
+constexpr int kExtItems = (kTileH + 2) * kExtNW;  // (row, word) items of a band
+static_assert(kExtItems <= kThreads, "one item per thread");
+typedef __attribute__((address_space(1))) const uint8_t gbyte_t;
+typedef __attribute__((address_space(1))) const unsigned long long gu64_t;
+
+// The NG level words of this thread's item of the band starting at row y0 (rows
+// y0-1 .. y0+16, clamped): 8-byte words inside [0, w) with one load (x0 is a
+// multiple of 64 and rows are 64-byte aligned), the others from clamped bytes.
+template <int NG>
+__device__ __forceinline__ void ext_fetch(const OctaveDesc& O, int x0, int y0,
+                                          unsigned long long (&u)[NG]) {
+    const int i = threadIdx.x;
+    if (i >= kExtItems) return;
+    const int w = O.w, h = O.h;
+    const int r = i / kExtNW, k = i - r * kExtNW;
+    const int y = min(max(y0 - 1 + r, 0), h - 1);
+    const int xw = x0 - 8 + 8 * k;
+    gbyte_t* const lv = (gbyte_t*)O.levels;
+    // fits 32 bits: images are at most 65536 x 65536 (larger ones are rejected)
+    const unsigned rowoff = (unsigned)y * (unsigned)O.pitch;
+    if (xw >= 0 && xw + 8 <= w) {
+#pragma unroll
+        for (int l = 0; l < NG; l++)
+            u[l] = *(gu64_t*)(lv + (size_t)l * O.level_stride + rowoff + (unsigned)xw);
+    } else {
+#pragma unroll
+        for (int l = 0; l < NG; l++) {
+            unsigned long long v = 0;
+#pragma unroll 1
+            for (int b = 0; b < 8; b++)
+                v |= (unsigned long long)lv[(size_t)l * O.level_stride + rowoff +
+                                             (unsigned)min(max(xw + b, 0), w - 1)]
+                     << (8 * b);
+            u[l] = v;
+        }
+    }
+}
+
+// D = q[G_{l+1}] - q[G_l] for columns [x0-1, x0+65) of the band's 18 rows, with
+// q formed as (float)((double)v * (1/255)), equal to v / 255.0f for every byte
+// value; D is the same f32 subtraction the reference makes.
+template <int NL, int NG>
+__device__ __forceinline__ void ext_stage(const unsigned long long (&u)[NG],
+                                          float (*D)[kTileH + 2][kTileW + 2]) {
+    const int i = threadIdx.x;
+    if (i >= kExtItems) return;
+    const int r = i / kExtNW, k = i - r * kExtNW;
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const int c = 8 * k + b - 7;  // column x0-1+c
+        if (c >= 0 && c < kTileW + 2) {
+            float qlo = (float)((double)(unsigned)(u[0] >> (8 * b) & 0xffu) * (1.0 / 255.0));
+#pragma unroll
+            for (int l = 0; l < NL; l++) {
+                const float qhi =
+                    (float)((double)(unsigned)(u[l + 1] >> (8 * b) & 0xffu) * (1.0 / 255.0));
+                D[l][r][c] = qhi - qlo;
+                qlo = qhi;
+            }
+        }
+    }
+}
 
 template <int S>
-__global__ __launch_bounds__(kThreads) void k_extrema_tile(const OctaveDesc* __restrict__ oct,
-                                                           int noct,
-                                                           const ImageParams* __restrict__ params,
-                                                           unsigned long long* __restrict__ bits,
-                                                           unsigned* __restrict__ counts) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_extrema_tile(
+    const OctaveDesc* __restrict__ oct, int noct, const ImageParams* __restrict__ params,
+    unsigned long long* __restrict__ bits, unsigned* __restrict__ counts) {
     constexpr int NL = S + 2;  // DoG layers
     constexpr int NG = S + 3;  // Gaussian levels
-    constexpr int RH = kTileH + 2, RW = kTileW + 2;
-    __shared__ float D[NL][RH][RW];
+    constexpr int RW = kTileW + 2;
+    __shared__ float D[NL][kTileH + 2][RW];
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int j = find_by(oct, noct, tile, &OctaveDesc::row_begin);
     const OctaveDesc& O = oct[j];
@@ -225,87 +289,61 @@ __global__ __launch_bounds__(kThreads) void k_extrema_tile(const OctaveDesc* __r
     const int t = tile - O.row_begin;
     const int w = O.w, h = O.h, border = P.border;
     const int tiles_x = (w + kTileW - 1) / kTileW;
-    const int x0 = (t % tiles_x) * kTileW, y0 = (t / tiles_x) * kTileH;
-    // Stage D = q[G_{l+1}] - q[G_l] for columns [x0-1, x0+65) of rows
-    // [y0-1, y0+17) (clamped): 4-byte words starting at x0-4 (x0 is a multiple of
-    // 64, rows are 64-byte aligned); words inside [0, w) are read with one load.
-    // q is formed as (float)((double)v * (1/255)), equal to v / 255.0f for every
-    // byte value, and D is the same f32 subtraction the reference makes.
-    constexpr int NW = (kTileW + 8) / 4;  // words covering [x0-4, x0+68)
-    for (int i = threadIdx.x; i < RH * NW; i += kThreads) {
-        const int r = i / NW, k = i - r * NW;
-        const int y = min(max(y0 - 1 + r, 0), h - 1);
-        const int xw = x0 - 4 + 4 * k;
-        const bool whole = xw >= 0 && xw + 4 <= w;
-        unsigned u[NG];
-#pragma unroll
-        for (int l = 0; l < NG; l++) {
-            const uint8_t* row = O.levels + (long long)l * O.level_stride + (size_t)y * O.pitch;
-            if (whole) {
-                u[l] = *(const unsigned*)(row + xw);
-            } else {
-                u[l] = 0;
-#pragma unroll
-                for (int b = 0; b < 4; b++) u[l] |= (unsigned)row[min(max(xw + b, 0), w - 1)] << (8 * b);
-            }
-        }
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const int c = 4 * k + b - 3;  // column x0-1+c
-            if (c >= 0 && c < RW) {
-                float qlo = (float)((double)(u[0] >> (8 * b) & 0xffu) * (1.0 / 255.0));
-#pragma unroll
-                for (int l = 0; l < NL; l++) {
-                    const float qhi = (float)((double)(u[l + 1] >> (8 * b) & 0xffu) * (1.0 / 255.0));
-                    D[l][r][c] = qhi - qlo;
-                    qlo = qhi;
-                }
-            }
-        }
-    }
-    __syncthreads();
+    const int x0 = (t % tiles_x) * kTileW, ys = (t / tiles_x) * kExtStripRows;
+    const int nbands = min(kExtStripRows, h - ys + kTileH - 1) / kTileH;  // >= 1
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int x = x0 + lane;
     const bool col_ok = x >= border && x < w - border;
-    float hmx[3][NL], hmn[3][NL], dl[3][NL], dc[3][NL], dr[3][NL];
-    auto load_row = [&](int slot, int r) {
+    unsigned long long u[NG];
+    ext_fetch<NG>(O, x0, ys, u);
+#pragma unroll 1
+    for (int band = 0; band < nbands; band++) {
+        const int y0 = ys + band * kTileH;
+        if (band > 0) __syncthreads();  // the previous band's tests are done with D
+        ext_stage<NL, NG>(u, D);
+        if (band + 1 < nbands) ext_fetch<NG>(O, x0, y0 + kTileH, u);
+        __syncthreads();
+
+        float hmx[3][NL], hmn[3][NL], dl[3][NL], dc[3][NL], dr[3][NL];
+        auto load_row = [&](int slot, int r) {
 #pragma unroll
-        for (int l = 0; l < NL; l++) {
-            const float a = D[l][r][lane];
-            const float b = D[l][r][lane + 1];
-            const float c = D[l][r][lane + 2];
-            dl[slot][l] = a;
-            dc[slot][l] = b;
-            dr[slot][l] = c;
-            hmx[slot][l] = fmaxf(a, fmaxf(b, c));
-            hmn[slot][l] = fminf(a, fminf(b, c));
-        }
-    };
-    const int rbase = wave * 4;  // local D row of output row y0 + wave*4 is rbase + 1
-    load_row(0, rbase);
-    load_row(1, rbase + 1);
+            for (int l = 0; l < NL; l++) {
+                const float a = D[l][r][lane];
+                const float b = D[l][r][lane + 1];
+                const float c = D[l][r][lane + 2];
+                dl[slot][l] = a;
+                dc[slot][l] = b;
+                dr[slot][l] = c;
+                hmx[slot][l] = fmaxf(a, fmaxf(b, c));
+                hmn[slot][l] = fminf(a, fminf(b, c));
+            }
+        };
+        const int rbase = wave * 4;  // local D row of output row y0 + wave*4 is rbase + 1
+        load_row(0, rbase);
+        load_row(1, rbase + 1);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int prv = k % 3, cur = (k + 1) % 3, nxt = (k + 2) % 3;
-        load_row(nxt, rbase + k + 2);
-        const int y = y0 + wave * 4 + k;
-        const bool row_ok = y >= border && y < h - border;
+        for (int k = 0; k < 4; k++) {
+            const int prv = k % 3, cur = (k + 1) % 3, nxt = (k + 2) % 3;
+            load_row(nxt, rbase + k + 2);
+            const int y = y0 + wave * 4 + k;
+            const bool row_ok = y >= border && y < h - border;
 #pragma unroll
-        for (int s = 1; s <= S; s++) {
-            float mx = fmaxf(fmaxf(hmx[prv][s - 1], hmx[cur][s - 1]), hmx[nxt][s - 1]);
-            mx = fmaxf(mx, fmaxf(fmaxf(hmx[prv][s + 1], hmx[cur][s + 1]), hmx[nxt][s + 1]));
-            mx = fmaxf(mx, fmaxf(fmaxf(hmx[prv][s], hmx[nxt][s]), fmaxf(dl[cur][s], dr[cur][s])));
-            float mn = fminf(fminf(hmn[prv][s - 1], hmn[cur][s - 1]), hmn[nxt][s - 1]);
-            mn = fminf(mn, fminf(fminf(hmn[prv][s + 1], hmn[cur][s + 1]), hmn[nxt][s + 1]));
-            mn = fminf(mn, fminf(fminf(hmn[prv][s], hmn[nxt][s]), fminf(dl[cur][s], dr[cur][s])));
-            const float v = dc[cur][s];
-            const bool ext = row_ok && col_ok && (v > mx || v < mn);
-            const unsigned long long m = ballot64(ext);
-            if (lane == 0 && y < h) {
-                const long long slot_local = (long long)(s - 1) * h + y;
-                bits[O.bits_begin + slot_local * O.nwords + (x0 >> 6)] = m;
-                if (m) atomicAdd(&counts[O.slot_begin + slot_local], (unsigned)__popcll(m));
+            for (int s = 1; s <= S; s++) {
+                float mx = fmaxf(fmaxf(hmx[prv][s - 1], hmx[cur][s - 1]), hmx[nxt][s - 1]);
+                mx = fmaxf(mx, fmaxf(fmaxf(hmx[prv][s + 1], hmx[cur][s + 1]), hmx[nxt][s + 1]));
+                mx = fmaxf(mx, fmaxf(fmaxf(hmx[prv][s], hmx[nxt][s]), fmaxf(dl[cur][s], dr[cur][s])));
+                float mn = fminf(fminf(hmn[prv][s - 1], hmn[cur][s - 1]), hmn[nxt][s - 1]);
+                mn = fminf(mn, fminf(fminf(hmn[prv][s + 1], hmn[cur][s + 1]), hmn[nxt][s + 1]));
+                mn = fminf(mn, fminf(fminf(hmn[prv][s], hmn[nxt][s]), fminf(dl[cur][s], dr[cur][s])));
+                const float v = dc[cur][s];
+                const bool ext = row_ok && col_ok && (v > mx || v < mn);
+                const unsigned long long m = ballot64(ext);
+                if (lane == 0 && y < h) {
+                    const long long slot_local = (long long)(s - 1) * h + y;
+                    bits[O.bits_begin + slot_local * O.nwords + (x0 >> 6)] = m;
+                    if (m) atomicAdd(&counts[O.slot_begin + slot_local], (unsigned)__popcll(m));
+                }
             }
         }
     }

@@ -167,5 +167,7 @@ constexpr int kMaxHPad = 3576;
 constexpr int kSmallMaxTaps4 = 2800;
 constexpr int kSmallMaxRows = 128;
 constexpr int kScanTileElems = 4096;
+// Rows of one extremum-test strip (64 columns wide, tested as bands of 16 rows).
+constexpr int kExtStripRows = 64;
 
 }  // namespace sh
