@@ -86,9 +86,15 @@ __device__ __forceinline__ float atan2_bins(float y, float x) {
     return copysignf(r, y);
 }
 
-// v / 255 in f32 without a table: (float)((double)v * (1/255)) equals the IEEE
-// f32 division for every byte value.
-__device__ __forceinline__ float q255x(unsigned v) { return (float)((double)v * (1.0 / 255.0)); }
+// v / 255 in f32 without a table, for v an integer in [0, 255] held as f32:
+// y = v * c with c = RN(1/255), then one correction with the exact residual
+// v - 255 y (an FMA). Equal to the IEEE division v / 255.0f for every byte value
+// (checked exhaustively); four full-rate f32 operations with the conversion.
+__device__ __forceinline__ float q255f(float v) {
+    constexpr float c = 0x1.010102p-8f;
+    const float y = v * c;
+    return __builtin_fmaf(__builtin_fmaf(-y, 255.0f, v), c, y);
+}
 
 __device__ __forceinline__ void fill_q(float* q) {
     for (int i = threadIdx.x; i < 256; i += blockDim.x) q[i] = __fdiv_rn((float)i, 255.0f);
@@ -250,8 +256,8 @@ __device__ __forceinline__ void ext_fetch(const OctaveDesc& O, int x0, int y0,
 }
 
 // D = q[G_{l+1}] - q[G_l] for columns [x0-1, x0+65) of the band's 18 rows, with
-// q formed as (float)((double)v * (1/255)), equal to v / 255.0f for every byte
-// value; D is the same f32 subtraction the reference makes.
+// q formed by q255f, equal to v / 255.0f for every byte value; D is the same f32
+// subtraction the reference makes.
 template <int NL, int NG>
 __device__ __forceinline__ void ext_stage(const unsigned long long (&u)[NG],
                                           float (*D)[kTileH + 2][kTileW + 2]) {
@@ -262,11 +268,10 @@ __device__ __forceinline__ void ext_stage(const unsigned long long (&u)[NG],
     for (int b = 0; b < 8; b++) {
         const int c = 8 * k + b - 7;  // column x0-1+c
         if (c >= 0 && c < kTileW + 2) {
-            float qlo = (float)((double)(unsigned)(u[0] >> (8 * b) & 0xffu) * (1.0 / 255.0));
+            float qlo = q255f((float)(unsigned)(u[0] >> (8 * b) & 0xffu));
 #pragma unroll
             for (int l = 0; l < NL; l++) {
-                const float qhi =
-                    (float)((double)(unsigned)(u[l + 1] >> (8 * b) & 0xffu) * (1.0 / 255.0));
+                const float qhi = q255f((float)(unsigned)(u[l + 1] >> (8 * b) & 0xffu));
                 D[l][r][c] = qhi - qlo;
                 qlo = qhi;
             }
@@ -686,8 +691,8 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                     }
                 }
                 *reinterpret_cast<float4*>(&pq[v * kOS + 4 * wd]) =
-                    make_float4(q255x(u & 255u), q255x((u >> 8) & 255u), q255x((u >> 16) & 255u),
-                                q255x(u >> 24));
+                    make_float4(q255f((float)(u & 255u)), q255f((float)((u >> 8) & 255u)),
+                                q255f((float)((u >> 16) & 255u)), q255f((float)(u >> 24)));
             }
             // per column and row of the window: the sample pixel's offset in the
             // staged window, or -1 when it is outside [1, w-2] x [1, h-2]
@@ -717,8 +722,8 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                 const int xi = f2i_sat(roundf(xo + (float)dx));
                 if (yi < 1 || yi >= h - 1 || xi < 1 || xi >= w - 1) continue;
                 const uint8_t* p = g + (size_t)yi * pitch + xi;
-                gx = q255x(p[1]) - q255x(p[-1]);
-                gy = q255x(p[pitch]) - q255x(p[-pitch]);
+                gx = q255f((float)p[1]) - q255f((float)p[-1]);
+                gy = q255f((float)p[pitch]) - q255f((float)p[-pitch]);
             }
             const float fdx = (float)dx, fdy = (float)dy;
             const float m = __builtin_amdgcn_sqrtf(__builtin_fmaf(gx, gx, gy * gy)) *

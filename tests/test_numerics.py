@@ -1,0 +1,49 @@
+"""Exactness of the device kernels' f32 shortcuts, checked on the CPU with exact
+rational arithmetic (each f32 operation rounded once, as the hardware does).
+
+- q255f (kernels_detect.hip): v / 255 for a byte v as y = v * RN(1/255) followed
+  by one FMA correction. The extremum test, refinement inputs and orientation
+  gradients use it in place of the IEEE division the reference performs
+  (src/sift.rs:97,120: `as f32 / 255.0`), so it must equal that division for
+  all 256 byte values.
+"""
+from fractions import Fraction
+
+import numpy as np
+
+
+def _rn32(x: Fraction) -> float:
+    """Rounds an exact rational to the nearest f32 (ties to even)."""
+    c = np.float32(float(x))
+    cands = {np.nextafter(c, np.float32(-np.inf)), c, np.nextafter(c, np.float32(np.inf))}
+
+    def key(t):
+        bits = int(np.array(t, dtype=np.float32).view(np.uint32))
+        return (abs(Fraction(float(t)) - x), bits & 1)
+
+    return float(min(cands, key=key))
+
+
+def _q255f(v: int) -> float:
+    c = Fraction(float.fromhex("0x1.010102p-8"))
+    y = Fraction(_rn32(v * c))
+    r = Fraction(_rn32(Fraction(v) - y * 255))  # fma(-y, 255, v)
+    return _rn32(r * c + y)                      # fma(r, c, y)
+
+
+def test_q255f_constant_is_rounded_reciprocal():
+    assert _rn32(Fraction(1, 255)) == float.fromhex("0x1.010102p-8")
+
+
+def test_q255f_equals_ieee_division_for_every_byte():
+    ieee = np.arange(256, dtype=np.float32) / np.float32(255.0)
+    got = np.array([_q255f(v) for v in range(256)], dtype=np.float32)
+    assert np.array_equal(got.view(np.uint32), ieee.view(np.uint32))
+
+
+def test_plain_reciprocal_product_is_not_enough():
+    # the correction step matters: v * RN(1/255) alone differs on many bytes
+    c = Fraction(float.fromhex("0x1.010102p-8"))
+    ieee = np.arange(256, dtype=np.float32) / np.float32(255.0)
+    plain = np.array([_rn32(v * c) for v in range(256)], dtype=np.float32)
+    assert (plain != ieee).sum() > 100
