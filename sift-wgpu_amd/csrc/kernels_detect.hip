@@ -279,10 +279,12 @@ __device__ __forceinline__ void ext_stage(const unsigned long long (&u)[NG],
     }
 }
 
+// Occupancy target: the LDS bound (6 work-groups per CU for S <= 3).
 template <int S>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_extrema_tile(
-    const OctaveDesc* __restrict__ oct, int noct, const ImageParams* __restrict__ params,
-    unsigned long long* __restrict__ bits, unsigned* __restrict__ counts) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(S <= 3 ? 6 : 3)))
+void k_extrema_tile(const OctaveDesc* __restrict__ oct, int noct,
+                    const ImageParams* __restrict__ params, unsigned long long* __restrict__ bits,
+                    unsigned* __restrict__ counts) {
     constexpr int NL = S + 2;  // DoG layers
     constexpr int NG = S + 3;  // Gaussian levels
     constexpr int RW = kTileW + 2;
