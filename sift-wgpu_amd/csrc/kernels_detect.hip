@@ -86,6 +86,12 @@ __device__ __forceinline__ float atan2_bins(float y, float x) {
     return copysignf(r, y);
 }
 
+// A wave-uniform f32 value moved to a scalar register (the compiler cannot prove
+// that values read from LDS or computed in VALU from uniform inputs are uniform).
+__device__ __forceinline__ float uniform_f(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 // v / 255 in f32 without a table, for v an integer in [0, 255] held as f32:
 // y = v * c with c = RN(1/255), then one correction with the exact residual
 // v - 255 y (an FMA). Equal to the IEEE division v / 255.0f for every byte value
@@ -864,24 +870,29 @@ __device__ __forceinline__ unsigned long long* desc_bins(unsigned* hcopy, float 
     const unsigned ai = ((unsigned)(int)af) & 7u;
     const float dx = xb - xf, dy = yb - yf;
     da = ab - af;
-    const unsigned cw = (unsigned)(int)__builtin_fmaf(yf, 6.0f * kCellWords,
-                                                      __builtin_fmaf(xf, (float)kCellWords, 7.0f * kCellWords));
+    // cell (xf, yf) of the padded 6 x 6 grid: 7 * kCellWords + the offset below
+    const unsigned cw =
+        (unsigned)(int)__builtin_fmaf(yf, 6.0f * kCellWords, xf * (float)kCellWords);
     const float t0 = m * (1.0f - dx), t1 = m * dx;
     tt[0] = t0 * (1.0f - dy);
     tt[1] = t1 * (1.0f - dy);
     tt[2] = t0 * dy;
     tt[3] = t1 * dy;
-    return reinterpret_cast<unsigned long long*>(hcopy + (cw + ai + __umul24(ai & 1u, kTableWords + 1u)));
+    return reinterpret_cast<unsigned long long*>(hcopy + 7 * kCellWords +
+                                                 (cw + ai + __umul24(ai & 1u, kTableWords + 1u)));
 }
 
 // Adds four pairs of fixed-point sums (a[2k], a[2k+1]) at the spatial
-// neighbours of a bin pair address.
+// neighbours of a bin pair address, each rounded as (unsigned)(a + bias):
+// bias 0.5 for plain sums, 0 for sums that started at 0.5.
+template <int kHalfBiased = 0>
 __device__ __forceinline__ void desc_add_pairs(unsigned long long* base, const float (&a)[8]) {
     constexpr int off[4] = {0, kCellWords / 2, 6 * kCellWords / 2, 7 * kCellWords / 2};
+    constexpr float bias = kHalfBiased ? 0.0f : 0.5f;
 #pragma unroll
     for (int k = 0; k < 4; k++)
-        atomicAdd(base + off[k], ((unsigned long long)(unsigned)(a[2 * k + 1] + 0.5f) << 32) |
-                                     (unsigned)(a[2 * k] + 0.5f));
+        atomicAdd(base + off[k], ((unsigned long long)(unsigned)(a[2 * k + 1] + bias) << 32) |
+                                     (unsigned)(a[2 * k] + bias));
 }
 
 __device__ __forceinline__ void desc_scatter(unsigned* hcopy, float xb, float yb, float ab,
@@ -1153,13 +1164,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
         desc_bounds(G, bx0, bx1, by0, by1);
         const int nrows = G.so > 0.0f ? max(by1 - by0 + 1, 0) : 0;  // <= 63
         for (int pk = 0; pk < G.np; pk++) {
-            const float angle = pang[pk];
+            const float angle = uniform_f(pang[pk]);
             for (int i = tid; i < kHistCopies * kHistWords / 4; i += kThreads)
                 reinterpret_cast<uint4*>(hist)[i] = make_uint4(0u, 0u, 0u, 0u);
-            const float ca = pcos[pk], sa = psin[pk];
-            const float inv_bw = 1.0f / bw;
+            const float ca = uniform_f(pcos[pk]), sa = uniform_f(psin[pk]);
+            const float inv_bw = uniform_f(1.0f / bw);
             const float msa = -sa;  // -sin * px is computed as (-sin) * px, the same value
-            const float abin = angle * (4.0f / kPi);  // the peak angle in bins
+            const float abin = uniform_f(angle * (4.0f / kPi));  // the peak angle in bins
             if (tid < 64) {
                 // per-row sample intervals of the rotated window, prefix-summed
                 int lo = 0, cnt = 0;
@@ -1218,7 +1229,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                 // by one sample step along the row
                 const float cai = ca * inv_bw, msai = msa * inv_bw;
                 // consecutive samples of a thread often share their bin pair: their
-                // contributions are summed in registers and added to LDS once per run
+                // contributions are summed in registers, starting from 0.5 so that the
+                // rounding to fixed point is a plain conversion, and added to LDS once
+                // per run
                 unsigned long long* acc_base = nullptr;
                 float acc[8];
                 float u = __builtin_fmaf(cai, (float)(cx - R), rr.z);
@@ -1237,10 +1250,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                     float tt[4], da;
                     unsigned long long* const base = desc_bins(hw, u + 1.5f, v + 1.5f, ab, m, tt, da);
                     if (base != acc_base) {  // a new bin pair: add the finished sums
-                        if (acc_base) desc_add_pairs(acc_base, acc);
+                        if (acc_base) desc_add_pairs<1>(acc_base, acc);
                         acc_base = base;
 #pragma unroll
-                        for (int q8 = 0; q8 < 8; q8++) acc[q8] = 0.0f;
+                        for (int q8 = 0; q8 < 8; q8++) acc[q8] = 0.5f;
                     }
                     const float wa0 = 1.0f - da;
 #pragma unroll
@@ -1249,7 +1262,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
                         acc[2 * q4 + 1] = __builtin_fmaf(tt[q4], da, acc[2 * q4 + 1]);
                     }
                 }
-                if (acc_base) desc_add_pairs(acc_base, acc);
+                if (acc_base) desc_add_pairs<1>(acc_base, acc);
             }
             __syncthreads();
             if (tid < 64)
