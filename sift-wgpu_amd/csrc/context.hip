@@ -636,7 +636,8 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             H.tiles = H.tiles_x * ((h + 3) / 4);
             hsj.push_back(H);
             hs_taps4 = std::max(hs_taps4, n4);
-        } else if ((hrow + (hrow >> 5) + 1) * 4 * (int)sizeof(float) <= 65536) {
+        } else if ((hrow + 8 + ((hrow + 8) >> (px == 8 ? 3 : 5)) + 1) * 4 * (int)sizeof(float) <=
+                   65536) {  // the 4 staged rows' LDS (padded as k_blur_h<px> pads them)
             H.tiles_x = (w + 64 * px - 1) / (64 * px);
             H.tiles = H.tiles_x * ((h + 3) / 4);
             (px == 8 ? hj : hj2).push_back(H);

@@ -53,18 +53,20 @@ __device__ __forceinline__ const ctap_t* as_const_taps(const float* p) {
 
 // ---------------------------------------------------------------------------
 // Horizontal pass. Tile = (64*PX) columns x 4 rows; wave w computes row y0+w;
-// lane l computes PX consecutive outputs. The row segment [x0-r, x0+64*PX+pad)
-// is staged in LDS as f32 with clamp-to-edge indexing (the reference's edge
-// rule), with pad words so that the lanes' reads, PX elements apart, fall in
-// distinct banks: PX = 8 puts one pad word after every 8 elements (element c at
-// c + c/8, lane stride 9 words), which also makes element base + k of every
-// lane sit at the same offset k + k/8 from the lane's first element; PX = 2 puts
-// one after every 32 (c + c/32).
+// lane l computes PX consecutive outputs. The row segment from the aligned word
+// at or before x0-r (d = (x0-r) mod 4 elements early) is staged in LDS as f32
+// with clamp-to-edge indexing (the reference's edge rule), one word of 4 pixels
+// per lane and step, with pad words so that the lanes' reads, PX elements apart,
+// fall in distinct banks: PX = 8 puts one pad word after every 8 elements
+// (element e at e + e/8, lane stride 9 words), PX = 2 one after every 32
+// (e + e/32). Either way a word's 4 pixels land in 4 consecutive slots.
 // For PX = 8 and the tap counts of the default scale schedule (7 to 15 taps) the
-// taps are fully unrolled: the PX + n - 1 window inputs are read once and every
-// tap is a scalar register. Otherwise a register window of PX+4 inputs slides
-// over the taps in blocks of 4; taps are zero-padded to a multiple of 4 (p * 0
-// added to a non-negative sum leaves it unchanged), except that a single
+// taps are fully unrolled: the PX + n - 1 window inputs, elements base + d + k,
+// sit at offsets d + k + (d + k)/8 from the lane's first slot (the same for every
+// lane), so they are read once with immediate offsets, one code path per d, and
+// every tap is a scalar register. Otherwise a register window of PX+4 inputs
+// slides over the taps in blocks of 4; taps are zero-padded to a multiple of 4
+// (p * 0 added to a non-negative sum leaves it unchanged), except that a single
 // leftover tap is applied on its own, and are wave-uniform (scalar registers).
 // ---------------------------------------------------------------------------
 template <int PX>
@@ -72,16 +74,17 @@ __host__ __device__ __forceinline__ int hpad(int c) {
     return PX == 8 ? c + (c >> 3) : c + (c >> 5);
 }
 
-// n taps over the window row[b8 + k + k/8], k < PX + n - 1 (b8 = hpad<8> of the
-// lane's first element), in tap order per output; the first product is the
-// first partial sum (0 + p * w0 is p * w0: p * w0 is never -0).
-template <int PX, int NT>
+// NT taps over the window elements base + D + k, k < PX + NT - 1, at
+// p[D + k + (D + k)/8] (p = the slot of element base, base a multiple of 8), in
+// tap order per output; the first product is the first partial sum (0 + p * w0
+// is p * w0: p * w0 is never -0).
+template <int PX, int NT, int D>
 __device__ __forceinline__ void hconv_fixed(const float* __restrict__ p, const ctap_t* taps,
                                             float (&acc)[PX]) {
     constexpr int NW = PX + NT - 1;
     float win[NW];
 #pragma unroll
-    for (int k = 0; k < NW; k++) win[k] = p[k + (k >> 3)];
+    for (int k = 0; k < NW; k++) win[k] = p[D + k + ((D + k) >> 3)];
     float wk[NT];
 #pragma unroll
     for (int k = 0; k < NT; k++) wk[k] = taps[k];
@@ -91,6 +94,17 @@ __device__ __forceinline__ void hconv_fixed(const float* __restrict__ p, const c
     for (int k = 1; k < NT; k++)
 #pragma unroll
         for (int q = 0; q < PX; q++) acc[q] = mul_add_exact(acc[q], win[q + k], wk[k]);
+}
+
+template <int NT>
+__device__ __forceinline__ void hconv_fixed_d(int d, const float* __restrict__ p,
+                                              const ctap_t* taps, float (&acc)[8]) {
+    switch (d) {
+        case 0: hconv_fixed<8, NT, 0>(p, taps, acc); break;
+        case 1: hconv_fixed<8, NT, 1>(p, taps, acc); break;
+        case 2: hconv_fixed<8, NT, 2>(p, taps, acc); break;
+        default: hconv_fixed<8, NT, 3>(p, taps, acc); break;
+    }
 }
 
 // Any tap count: taps in blocks of 4 (taps beyond ntaps are zero, <= ntaps_pad).
@@ -140,39 +154,48 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
     const int r = J.ntaps >> 1;
     const int npad = J.ntaps_pad;
     const int rowlen = TW + npad + 8;
-    const int rowstride = hpad<PX>(rowlen) + 1;
+    const int rowstride = hpad<PX>(rowlen + 8) + 1;
 
-    // Wave w stages row y0+w, columns [x0-r, x0-r+rowlen) (clamped) as f32, and
-    // reads only that row, so the waves need no work-group barrier. Aligned
-    // 4-byte words fully inside [0, w) are read with one load; words at the
-    // image edges (and unaligned sources) fall back to clamped bytes.
+    // Wave w stages row y0+w, the words [ws, ws + 4 * nwords) with ws the aligned
+    // word start at or before x0-r (clamped) as f32, and reads only that row, so
+    // the waves need no work-group barrier. Aligned 4-byte words fully inside
+    // [0, w) are read with one load; words at the image edges (and unaligned
+    // sources) fall back to clamped bytes. The first kHWordsUnrolled words of a
+    // lane are all loaded before any is written, so their loads overlap.
     const int xs = x0 - r;
     const int ws = xs & ~3;                      // first word start (may be < 0)
-    const int nwords = (xs + rowlen - ws + 3) >> 2;
+    const int d = xs - ws;                       // element base + k is staged element base + d + k
+    const int nwords = (d + rowlen + 3) >> 2;
     const bool aligned = (((uintptr_t)J.src | (uintptr_t)J.src_pitch) & 3) == 0;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int y = y0 + wave;
     if (y >= h) return;
     const uint8_t* src = J.src + (size_t)y * J.src_pitch;
     float* dstrow = lds + wave * rowstride;
-    for (int k = lane; k < nwords; k += 64) {
+    auto fetch = [&](int k) -> unsigned {
         const int xw = ws + 4 * k;
-        const int c0 = xw - xs;
-        float v[4];
-        if (aligned && xw >= 0 && xw + 4 <= w) {
-            const unsigned u = *(const unsigned*)(src + xw);
-            v[0] = (float)(u & 0xff);
-            v[1] = (float)((u >> 8) & 0xff);
-            v[2] = (float)((u >> 16) & 0xff);
-            v[3] = (float)(u >> 24);
-        } else {
+        if (aligned && xw >= 0 && xw + 4 <= w) return *(const unsigned*)(src + xw);
+        unsigned u = 0;
 #pragma unroll
-            for (int b = 0; b < 4; b++) v[b] = (float)src[min(max(xw + b, 0), w - 1)];
-        }
+        for (int b = 0; b < 4; b++) u |= (unsigned)src[min(max(xw + b, 0), w - 1)] << (8 * b);
+        return u;
+    };
+    auto put = [&](int k, unsigned u) {  // pixels 4k..4k+3: 4 consecutive slots
+        float* dp = dstrow + hpad<PX>(4 * k);
+        dp[0] = (float)(u & 0xff);
+        dp[1] = (float)((u >> 8) & 0xff);
+        dp[2] = (float)((u >> 16) & 0xff);
+        dp[3] = (float)(u >> 24);
+    };
+    constexpr int KU = PX == 8 ? 3 : 1;  // words per lane of a row of the default kernels
+    unsigned uw[KU];
 #pragma unroll
-        for (int b = 0; b < 4; b++)
-            if (c0 + b >= 0 && c0 + b < rowlen) dstrow[hpad<PX>(c0 + b)] = v[b];
-    }
+    for (int i = 0; i < KU; i++)
+        if (lane + 64 * i < nwords) uw[i] = fetch(lane + 64 * i);
+#pragma unroll
+    for (int i = 0; i < KU; i++)
+        if (lane + 64 * i < nwords) put(lane + 64 * i, uw[i]);
+    for (int k = lane + 64 * KU; k < nwords; k += 64) put(k, fetch(k));
     // the row's LDS writes are complete and visible to the wave before its reads
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -186,15 +209,15 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
         const float* p = row + hpad<8>(base);
         fixed = true;
         switch (J.ntaps) {
-            case 7: hconv_fixed<8, 7>(p, taps, acc); break;
-            case 9: hconv_fixed<8, 9>(p, taps, acc); break;
-            case 11: hconv_fixed<8, 11>(p, taps, acc); break;
-            case 13: hconv_fixed<8, 13>(p, taps, acc); break;
-            case 15: hconv_fixed<8, 15>(p, taps, acc); break;
+            case 7: hconv_fixed_d<7>(d, p, taps, acc); break;
+            case 9: hconv_fixed_d<9>(d, p, taps, acc); break;
+            case 11: hconv_fixed_d<11>(d, p, taps, acc); break;
+            case 13: hconv_fixed_d<13>(d, p, taps, acc); break;
+            case 15: hconv_fixed_d<15>(d, p, taps, acc); break;
             default: fixed = false;
         }
     }
-    if (!fixed) hconv_blocks<PX>(row, base, taps, J.ntaps, acc);
+    if (!fixed) hconv_blocks<PX>(row, base + d, taps, J.ntaps, acc);
 
     const int x = x0 + base;
     if (x >= w) return;
@@ -748,7 +771,7 @@ void launch_blur_h(const BlurJob* d_jobs, int njobs, int total_tiles, int max_pa
     if (total_tiles <= 0) return;
     const int rowlen = 64 * px + max_pad + 8;
     const size_t lds =
-        (size_t)kHTileH * ((px == 8 ? hpad<8>(rowlen) : hpad<2>(rowlen)) + 1) * sizeof(float);
+        (size_t)kHTileH * ((px == 8 ? hpad<8>(rowlen + 8) : hpad<2>(rowlen + 8)) + 1) * sizeof(float);
     if (px == 8)
         hipLaunchKernelGGL(k_blur_h<8>, dim3(total_tiles), dim3(kThreads), lds, s, d_jobs, njobs);
     else
