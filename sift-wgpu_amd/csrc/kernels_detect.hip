@@ -512,12 +512,11 @@ struct DogView {
     const uint8_t* levels;
     long long stride;
     int pitch, w, h;
-    const float* q;
     __device__ __forceinline__ float at(int l, int x, int y) const {
         x = min(max(x, 0), w - 1);
         y = min(max(y, 0), h - 1);
         const uint8_t* g0 = levels + (long long)l * stride + (size_t)y * pitch + x;
-        return q[g0[stride]] - q[g0[0]];
+        return q255f((float)g0[stride]) - q255f((float)g0[0]);
     }
 };
 
@@ -528,15 +527,12 @@ __global__ __launch_bounds__(kThreads) void k_refine(const OctaveDesc* __restric
                                                      Keypoint* __restrict__ out,
                                                      int* __restrict__ valid,
                                                      float* __restrict__ cand_resp) {
-    __shared__ float q[256];
-    fill_q(q);
-    __syncthreads();
     const int n = (int)min(*total, (unsigned)cap);
     for (int c = blockIdx.x * kThreads + threadIdx.x; c < n; c += gridDim.x * kThreads) {
         const Cand cd = cand[c];
         const OctaveDesc& O = oct[cd.job];
         const ImageParams& P = params[O.img];
-        DogView V{O.levels, O.level_stride, O.pitch, O.w, O.h, q};
+        DogView V{O.levels, O.level_stride, O.pitch, O.w, O.h};
         const int nd = P.num_intervals + 2;  // DoG layers per octave
         const float scale0 = pow2f(O.octave);
         // kp.x / 2^o of the candidate ((x + 0.5) * 2^o), `as i32` (src/sift.rs:309-313)

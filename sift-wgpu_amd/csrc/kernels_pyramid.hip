@@ -47,6 +47,8 @@ __device__ __forceinline__ float q255(unsigned v) { return __fdiv_rn((float)v, 2
 // by kernels: reading them through the constant address space lets the
 // wave-uniform tap loads go through scalar loads instead of vector loads.
 typedef __attribute__((address_space(4))) const float ctap_t;
+// Global memory (source rows): loads through it are global, not flat, loads.
+typedef __attribute__((address_space(1))) const uint8_t gbyte_t;
 __device__ __forceinline__ const ctap_t* as_const_taps(const float* p) {
     return (const ctap_t*)(p);
 }
@@ -170,11 +172,12 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int y = y0 + wave;
     if (y >= h) return;
-    const uint8_t* src = J.src + (size_t)y * J.src_pitch;
+    gbyte_t* src = (gbyte_t*)J.src + (size_t)y * J.src_pitch;
     float* dstrow = lds + wave * rowstride;
     auto fetch = [&](int k) -> unsigned {
         const int xw = ws + 4 * k;
-        if (aligned && xw >= 0 && xw + 4 <= w) return *(const unsigned*)(src + xw);
+        if (aligned && xw >= 0 && xw + 4 <= w)
+            return *(__attribute__((address_space(1))) const unsigned*)(src + xw);
         unsigned u = 0;
 #pragma unroll
         for (int b = 0; b < 4; b++) u |= (unsigned)src[min(max(xw + b, 0), w - 1)] << (8 * b);
@@ -338,8 +341,6 @@ __device__ __forceinline__ void vconv_blocks(const uint8_t* col, int pitch, int 
 // are streamed once in increasing order, and input row i adds tap i - q to
 // output row q -- per output, the taps in tap order. The first product is the
 // first partial sum (p * w0 is never -0).
-typedef __attribute__((address_space(1))) const uint8_t gbyte_t;
-
 template <int RV, int NT>
 __device__ __forceinline__ void vconv_fixed(const uint8_t* src, unsigned x, int pitch, int y0,
                                             int h, const ctap_t* taps, float (&acc)[RV][4]) {
