@@ -491,6 +491,13 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     int S0 = (int)B.params[0].num_intervals;
     bool tile_ext = extrema_tile_supported(S0);
     for (int i = 1; i < n; i++) tile_ext = tile_ext && (int)B.params[i].num_intervals == S0;
+    // strips of kExtStripRows rows when the batch has enough 16-row bands to fill
+    // the GPU with them; single bands otherwise (single frames, small batches)
+    long long bands16 = 0;
+    for (int i = 0; i < n; i++)
+        for (const OctShape& s : shapes[i])
+            if (s.w > 0 && s.h > 0) bands16 += (long long)((s.w + 63) / 64) * ((s.h + 15) / 16);
+    const int strip_rows = bands16 >= kExtFillBands ? kExtStripRows : 16;
     int total_tiles = 0;
     std::vector<OctaveDesc> octs;
     std::vector<int> img_slot_begin(n + 1);
@@ -515,7 +522,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             d.bits_begin = nbits;
             d.nwords = (s.w + 63) / 64;
             d.row_begin = tile_ext ? total_tiles : total_rows;
-            total_tiles += ((s.w + 63) / 64) * ((s.h + kExtStripRows - 1) / kExtStripRows);
+            total_tiles += ((s.w + 63) / 64) * ((s.h + strip_rows - 1) / strip_rows);
             nslots += S * s.h;
             nbits += (long long)S * s.h * d.nwords;
             total_rows += s.h;
@@ -894,7 +901,8 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         hipEvent_t ev = nullptr;
         prof_begin(K_EXTREMA, ext_bytes, &ev);
         if (tile_ext)
-            launch_extrema_tile(S0, d_oct, noct, total_tiles, d_par, (unsigned long long*)AP(bits_off),
+            launch_extrema_tile(S0, strip_rows, d_oct, noct, total_tiles, d_par,
+                                (unsigned long long*)AP(bits_off),
                                 d_counts, nslots, st);
         else
             launch_extrema(d_oct, noct, total_rows, d_par, (unsigned long long*)AP(bits_off), d_counts,
@@ -940,6 +948,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         // everything the launch sequence depends on
         std::vector<uint64_t> key = {(uint64_t)(uintptr_t)A, blob_dev_off, (uint64_t)B.pyramid_only,
                                      (uint64_t)B.want_stages, (uint64_t)tile_ext, (uint64_t)S0,
+                                     (uint64_t)strip_rows,
                                      (uint64_t)noct, (uint64_t)total_tiles, (uint64_t)total_rows,
                                      (uint64_t)nslots, (uint64_t)cap, (uint64_t)kcap, (uint64_t)n,
                                      oct_blob, par_blob, isb_blob, bits_off, counts_off, offs_off,

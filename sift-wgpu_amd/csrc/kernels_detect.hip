@@ -205,8 +205,8 @@ __global__ __launch_bounds__(kThreads) void k_extrema(const OctaveDesc* __restri
 }
 
 // ---------------------------------------------------------------------------
-// k_extrema_tile<S>: the same test on strips of 64 columns x kExtStripRows rows,
-// taken as bands of 16 rows. The S+2 DoG layers of a band plus a 1-pixel halo are
+// k_extrema_tile<S>: the same test on strips of 64 columns x strip_rows rows
+// (16 or kExtStripRows), taken as bands of 16 rows. The S+2 DoG layers of a band plus a 1-pixel halo are
 // staged in LDS as f32 (each thread holds one 8-byte word of every Gaussian level
 // and writes q[G_{l+1}] - q[G_l] for its 8 pixels); the words of the next band
 // are loaded into registers before the current band is tested, so their latency
@@ -290,7 +290,7 @@ template <int S>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(S <= 3 ? 6 : 3)))
 void k_extrema_tile(const OctaveDesc* __restrict__ oct, int noct,
                     const ImageParams* __restrict__ params, unsigned long long* __restrict__ bits,
-                    unsigned* __restrict__ counts) {
+                    unsigned* __restrict__ counts, int strip_rows) {
     constexpr int NL = S + 2;  // DoG layers
     constexpr int NG = S + 3;  // Gaussian levels
     constexpr int RW = kTileW + 2;
@@ -302,8 +302,8 @@ void k_extrema_tile(const OctaveDesc* __restrict__ oct, int noct,
     const int t = tile - O.row_begin;
     const int w = O.w, h = O.h, border = P.border;
     const int tiles_x = (w + kTileW - 1) / kTileW;
-    const int x0 = (t % tiles_x) * kTileW, ys = (t / tiles_x) * kExtStripRows;
-    const int nbands = min(kExtStripRows, h - ys + kTileH - 1) / kTileH;  // >= 1
+    const int x0 = (t % tiles_x) * kTileW, ys = (t / tiles_x) * strip_rows;
+    const int nbands = min(strip_rows, h - ys + kTileH - 1) / kTileH;  // >= 1
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int x = x0 + lane;
@@ -1384,7 +1384,7 @@ void launch_extrema(const OctaveDesc* d_oct, int noct, int total_rows, const Ima
 
 bool extrema_tile_supported(int S) { return S >= 1 && S <= 6; }
 
-void launch_extrema_tile(int S, const OctaveDesc* d_oct, int noct, int total_tiles,
+void launch_extrema_tile(int S, int strip_rows, const OctaveDesc* d_oct, int noct, int total_tiles,
                          const ImageParams* d_params, unsigned long long* d_bits, unsigned* d_counts,
                          int nslots, hipStream_t s) {
     // counts[nslots] is the descriptor's wide-window keypoint count
@@ -1393,7 +1393,7 @@ void launch_extrema_tile(int S, const OctaveDesc* d_oct, int noct, int total_til
     switch (S) {
 #define SH_EXT(SS) \
         case SS: hipLaunchKernelGGL(k_extrema_tile<SS>, dim3(total_tiles), dim3(kThreads), 0, s, d_oct, \
-                                    noct, d_params, d_bits, d_counts); break;
+                                    noct, d_params, d_bits, d_counts, strip_rows); break;
         SH_EXT(1) SH_EXT(2) SH_EXT(3) SH_EXT(4) SH_EXT(5) SH_EXT(6)
 #undef SH_EXT
         default: break;

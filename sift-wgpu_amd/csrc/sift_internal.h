@@ -133,7 +133,7 @@ void launch_extrema(const OctaveDesc* d_oct, int noct, int total_rows, const Ima
 // Tiled extremum scan (64x16 tiles) for S in [1, 6]: zeroes the row-slot
 // counts, then one workgroup per tile; OctaveDesc.row_begin holds the tile prefix.
 bool extrema_tile_supported(int S);
-void launch_extrema_tile(int S, const OctaveDesc* d_oct, int noct, int total_tiles,
+void launch_extrema_tile(int S, int strip_rows, const OctaveDesc* d_oct, int noct, int total_tiles,
                          const ImageParams* d_params, unsigned long long* d_bits, unsigned* d_counts,
                          int nslots, hipStream_t s);
 void launch_scan_dev(const unsigned* d_in, int n_max, const unsigned* n_dev, unsigned* d_out,
@@ -167,7 +167,10 @@ constexpr int kMaxHPad = 3576;
 constexpr int kSmallMaxTaps4 = 2800;
 constexpr int kSmallMaxRows = 128;
 constexpr int kScanTileElems = 4096;
-// Rows of one extremum-test strip (64 columns wide, tested as bands of 16 rows).
+// Rows of one extremum-test strip (64 columns wide, tested as bands of 16 rows):
+// kExtStripRows when a launch has at least kExtFillBands bands of 16 rows (enough
+// strips to fill the GPU several times over), else one band per strip.
 constexpr int kExtStripRows = 64;
+constexpr int kExtFillBands = 18432;
 
 }  // namespace sh

@@ -141,6 +141,21 @@ def test_batch_equals_alone():
         assert alone[1].tobytes() == together[i][1].tobytes(), i
 
 
+def test_large_batch_equals_alone():
+    """A batch large enough for the extremum test's multi-band strips (heights that
+    are not multiples of the strip or the band included) gives each image the
+    result it gets alone, where the strips are single bands."""
+    shapes = [(1920, 1080)] * 6 + [(1000, 1003), (1366, 777)]
+    imgs = [imgen.generic_image(w, h, seed=200 + i) for i, (w, h) in enumerate(shapes)]
+    params = [sift_hip.Sift(1.6, imgen.octaves_for(w, h), 3, 0.5, 0.04, 10.0) for (w, h) in shapes]
+    together = sift_hip.Sift.default().detect_batch(imgs, params)
+    for i, img in enumerate(imgs):
+        alone = params[i].detect_and_compute(img)
+        assert len(alone[0]) > 0, i
+        assert alone[0].tobytes() == together[i][0].tobytes(), i
+        assert alone[1].tobytes() == together[i][1].tobytes(), i
+
+
 def test_high_contrast_threshold():
     # tests/integration_test.rs:228-281
     img, octv, _ = imgen.config_image("512x512")
