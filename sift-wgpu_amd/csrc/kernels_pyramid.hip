@@ -21,8 +21,19 @@ constexpr int kVTileW = 256;  // 64 lanes x 4 px
 constexpr int kVRows = 4;     // rows per lane in k_blur_v
 constexpr int kVTileH = 4 * kVRows;  // 4 waves x kVRows rows
 
+// The job of a tile (jobs[j].tile_begin <= tile < jobs[j+1].tile_begin); every
+// launch's grid is its total tile count. With more than 4 jobs the proportional
+// guess tile * njobs / total is checked first: it is exact when the jobs have
+// equal tile counts (a batch of equal frames), and then the lookup is two
+// independent loads instead of a chain of dependent ones; otherwise, and for
+// other layouts, a binary search.
 template <typename Job>
 __device__ __forceinline__ int find_job(const Job* __restrict__ jobs, int njobs, int tile) {
+    if (njobs > 4) {
+        const int g = (int)(((long long)tile * njobs) / (long long)gridDim.x);
+        if (g < njobs && jobs[g].tile_begin <= tile && (g + 1 == njobs || jobs[g + 1].tile_begin > tile))
+            return g;
+    }
     int lo = 0, hi = njobs - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
