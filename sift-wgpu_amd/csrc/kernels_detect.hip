@@ -708,42 +708,66 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
             }
         }
         __syncthreads();
-        const float inv_side = 1.0f / (float)side;
-        for (int it = 0; it < iters; it++) {
-            const int i = lane + 64 * it;
-            if (i >= nsamp) break;
-            const int iy = (int)(((float)i + 0.5f) * inv_side);
-            const int dy = iy - R, dx = i - iy * side - R;
-            float gx, gy;
+        // lane l takes the samples [l C, (l+1) C) in raster order, C = ceil(n / 64):
+        // lanes sit far apart in the window, and a lane's consecutive samples are
+        // neighbours that often share a bin, so their fixed-point values are
+        // summed in registers (integer sums: the histogram is the same as with
+        // one add per sample) and added to LDS once per run
+        const int i0 = min(lane * iters, nsamp), i1 = min(i0 + iters, nsamp);
+        const int iy0 = (int)(((float)i0 + 0.5f) * (1.0f / (float)side));
+        int dy = iy0 - R, dx = i0 - iy0 * side - R;
+        unsigned long long run = 0;
+        int run_bin = -1;
+        unsigned long long* const hrow = hist[lane & 3];
+        for (int i = i0; i < i1; i++) {
+            float gx = 0.0f, gy = 0.0f;
+            bool in = true;
             if (lds) {
                 const int ox = ocx[dx + R], oy = ocy[dy + R];
-                if ((ox | oy) < 0) continue;
-                const float* pp = &pq[oy + ox];
-                gx = pp[1] - pp[-1];
-                gy = pp[kOS] - pp[-kOS];
+                in = (ox | oy) >= 0;
+                if (in) {
+                    const float* pp = &pq[oy + ox];
+                    gx = pp[1] - pp[-1];
+                    gy = pp[kOS] - pp[-kOS];
+                }
             } else {
                 const int yi = f2i_sat(roundf(yo + (float)dy));
                 const int xi = f2i_sat(roundf(xo + (float)dx));
-                if (yi < 1 || yi >= h - 1 || xi < 1 || xi >= w - 1) continue;
-                const uint8_t* p = g + (size_t)yi * pitch + xi;
-                gx = q255f((float)p[1]) - q255f((float)p[-1]);
-                gy = q255f((float)p[pitch]) - q255f((float)p[-pitch]);
+                in = yi >= 1 && yi < h - 1 && xi >= 1 && xi < w - 1;
+                if (in) {
+                    const uint8_t* p = g + (size_t)yi * pitch + xi;
+                    gx = q255f((float)p[1]) - q255f((float)p[-1]);
+                    gy = q255f((float)p[pitch]) - q255f((float)p[-pitch]);
+                }
             }
-            const float fdx = (float)dx, fdy = (float)dy;
-            const float m = __builtin_amdgcn_sqrtf(__builtin_fmaf(gx, gx, gy * gy)) *
-                            __builtin_amdgcn_exp2f(__builtin_fmaf(kq, __builtin_fmaf(fdx, fdx, fdy * fdy), lf));
-            const float ang = atan2_grad(gy, gx);
-            const float an = ang < 0.0f ? ang + kTwoPi : ang;
-            // (an * 36) / 2pi with the reference's two roundings: gradients along the
-            // axes land exactly on bin edges, where the rounding decides the bin
-            int bin = (int)floorf(an * 36.0f / kTwoPi);
-            bin = bin >= 36 ? bin - 36 : bin;
-            // m * 2^40 as a u64: the high word is exact (power-of-two scaling and
-            // floor), the remainder x - hi * 2^32 is exact (Sterbenz), both truncate
-            const float hi = floorf(m * 2.3283064365386963e-10f);
-            const float lo = __builtin_fmaf(-hi, 4294967296.0f, m);
-            atomicAdd(&hist[lane & 3][bin], ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+            if (in) {
+                const float fdx = (float)dx, fdy = (float)dy;
+                const float m = __builtin_amdgcn_sqrtf(__builtin_fmaf(gx, gx, gy * gy)) *
+                                __builtin_amdgcn_exp2f(__builtin_fmaf(kq, __builtin_fmaf(fdx, fdx, fdy * fdy), lf));
+                const float ang = atan2_grad(gy, gx);
+                const float an = ang < 0.0f ? ang + kTwoPi : ang;
+                // (an * 36) / 2pi with the reference's two roundings: gradients along the
+                // axes land exactly on bin edges, where the rounding decides the bin
+                int bin = (int)floorf(an * 36.0f / kTwoPi);
+                bin = bin >= 36 ? bin - 36 : bin;
+                // m * 2^40 as a u64: the high word is exact (power-of-two scaling and
+                // floor), the remainder x - hi * 2^32 is exact (Sterbenz), both truncate
+                const float hi = floorf(m * 2.3283064365386963e-10f);
+                const float lo = __builtin_fmaf(-hi, 4294967296.0f, m);
+                const unsigned long long v = ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+                if (bin != run_bin) {
+                    if (run_bin >= 0) atomicAdd(&hrow[run_bin], run);
+                    run_bin = bin;
+                    run = 0;
+                }
+                run += v;
+            }
+            if (++dx > R) {
+                dx = -R;
+                dy++;
+            }
         }
+        if (run_bin >= 0) atomicAdd(&hrow[run_bin], run);
         __syncthreads();
         float hv = 0.0f;
         if (lane < 36) {
