@@ -616,12 +616,12 @@ __global__ __launch_bounds__(kThreads) void k_refine(const OctaveDesc* __restric
 // k_orient: one wave (workgroup of 64) per refined keypoint, grid-stride
 // (src/sift.rs:605-726). For R <= 16 the window's rows and columns of the
 // Gaussian level (every pixel the gradient of a sample can touch) are staged
-// in LDS as q = v/255 floats. Lane l takes samples l, l+64, ... of the
-// (2R+1)^2 window in raster order; each adds mag * exp(-(dx^2+dy^2)/denom) to
-// bin floor(theta * 36 / 2pi) of a u64 fixed-point (2^40) histogram, one of 4
-// copies by lane (neighbouring lanes, whose samples are neighbours, hit
-// different copies; integer adds: order-independent, deterministic, and as
-// fine as the f32 contributions themselves). Lanes 0..35 then hold one bin
+// in LDS as q = v/255 floats. Lane l takes one contiguous chunk of the
+// (2R+1)^2 window in raster order; each sample adds mag * exp(-(dx^2+dy^2)/denom)
+// to bin floor(theta * 36 / 2pi) of a u64 fixed-point (2^40) histogram, one of 4
+// copies by lane, with runs of same-bin samples summed in registers first
+// (integer adds: order-independent, deterministic, and as fine as the f32
+// contributions themselves). Lanes 0..35 then hold one bin
 // each for the two [1,1,1]/3 smoothing passes, the max and the peak test; a
 // ballot ranks the peaks in bin order.
 // ---------------------------------------------------------------------------
