@@ -202,6 +202,8 @@ const LanczosRef& Context::lanczos_for(int in, int out) {
     R.left_off = tables.append(left.data(), left.size() * sizeof(int));
     R.count_off = tables.append(cnt.data(), cnt.size() * sizeof(int));
     R.w_off = tables.append(wts.data(), wts.size() * sizeof(float));
+    R.left = std::move(left);
+    R.count = std::move(cnt);
     return lanczos.emplace(key, R).first->second;
 }
 
@@ -344,14 +346,14 @@ struct LaunchRec {
 enum KernelId {
     K_BLUR_H = 0, K_BLUR_V, K_COPY, K_BLUR_H_GENERIC, K_RESIZE_V, K_RESIZE_H, K_EXTREMA, K_SCAN,
     K_COMPACT, K_REFINE, K_ORIENT, K_DESCRIPTOR, K_FINALIZE, K_BLUR_V_NARROW, K_BLUR_V_GENERIC,
-    K_BLUR_H2, K_BLUR_H_SMALL, K_BLUR_V_SMALL, K_LUMA, K_DESC_U8, K_COUNT
+    K_BLUR_H2, K_BLUR_H_SMALL, K_BLUR_V_SMALL, K_LUMA, K_DESC_U8, K_RESIZE_FUSED, K_COUNT
 };
 // Names as rocprofv3 reports the kernels (template arguments included).
 const char* const kKernelNames[K_COUNT] = {
     "k_blur_h<8>", "k_blur_v", "k_copy_level", "k_blur_h_generic", "k_resize_v", "k_resize_h",
     "k_extrema_tile", "k_scan", "k_compact", "k_refine", "k_orient", "k_descriptor",
     "k_finalize", "k_blur_v_narrow", "k_blur_v_generic", "k_blur_h<2>", "k_blur_h_small",
-    "k_blur_v_small", "k_luma", "k_desc_u8"};
+    "k_blur_v_small", "k_luma", "k_desc_u8", "k_resize_fused"};
 
 
 double blur_job_bytes(const BlurJob& j, int kind) {
@@ -366,6 +368,7 @@ double luma_job_bytes(const LumaJob& j) {
     return (double)j.w * j.h * (pixel_bytes((uint32_t)j.format) + 1);
 }
 double resize_job_bytes(const ResizeJob& j, int kind) {
+    if (kind == K_RESIZE_FUSED) return (double)j.in_w * j.in_h + (double)j.out_w * j.out_h;
     if (kind == K_RESIZE_V) return (double)j.in_w * j.in_h + (double)j.in_w * j.out_h * 4.0;
     return (double)j.in_w * j.out_h * 4.0 + (double)j.out_w * j.out_h;
 }
@@ -570,7 +573,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         maxO = std::max(maxO, (int)shapes[i].size());
         Lmax = std::max(Lmax, (int)B.params[i].num_intervals + 3);
     }
-    const size_t max_launches = (size_t)(1 + maxO * Lmax) * 10 + (size_t)maxO * 2 + 8;
+    const size_t max_launches = (size_t)(1 + maxO * Lmax) * 10 + (size_t)maxO * 3 + 8;
     const size_t max_jobs = (size_t)n * ((1 + (size_t)maxO * Lmax) * 2 + (size_t)maxO * 2);
     const size_t blob_bound = max_launches * 256 + max_jobs * std::max(sizeof(BlurJob), sizeof(ResizeJob)) +
                               octs.size() * sizeof(OctaveDesc) + (size_t)n * sizeof(ImageParams) +
@@ -752,7 +755,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             flush_blur();
         }
         // Lanczos3 to the next octave base (src/sift.rs:166-173)
-        std::vector<ResizeJob> rv, rh;
+        std::vector<ResizeJob> rv, rh, rf;
         for (int i = 0; i < n; i++) {
             if ((int)shapes[i].size() <= o + 1) continue;
             const OctShape& a = shapes[i][o];
@@ -772,6 +775,29 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             J.src_pitch = a.pitch;
             J.tmp_pitch = a.pitch;
             J.dst_pitch = b.pitch;
+            // both passes in one kernel when every tile's input span fits its LDS
+            bool fuse = true;
+            for (int ox0 = 0; ox0 < b.w && fuse; ox0 += 256) {
+                const int oxl = std::min(ox0 + 256, b.w) - 1;
+                const int lo4 = LH.left[ox0] & ~3;
+                const int hi = LH.left[oxl] + LH.count[oxl];
+                fuse = (int)round_up((size_t)(hi - lo4), 4) <= kResizeFusedSpan;
+            }
+            if (fuse) {
+                ResizeJob F = J;
+                F.left = (const int*)(T + LH.left_off);
+                F.count = (const int*)(T + LH.count_off);
+                F.weights = (const float*)(T + LH.w_off);
+                F.maxtaps = LH.maxtaps;
+                F.vleft = (const int*)(T + LV.left_off);
+                F.vcount = (const int*)(T + LV.count_off);
+                F.vweights = (const float*)(T + LV.w_off);
+                F.vmaxtaps = LV.maxtaps;
+                F.tiles_x = (b.w + 255) / 256;
+                F.tiles = F.tiles_x * ((b.h + kResizeRows - 1) / kResizeRows);
+                rf.push_back(F);
+                continue;
+            }
             ResizeJob V = J;
             V.left = (const int*)(T + LV.left_off);
             V.count = (const int*)(T + LV.count_off);
@@ -786,9 +812,10 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             H.weights = (const float*)(T + LH.w_off);
             H.maxtaps = LH.maxtaps;
             H.tiles_x = (b.w + 255) / 256;
-            H.tiles = H.tiles_x * ((b.h + 7) / 8);
+            H.tiles = H.tiles_x * ((b.h + kResizeRows - 1) / kResizeRows);
             rh.push_back(H);
         }
+        push_launch(K_RESIZE_FUSED, rf, 0, 0);
         push_launch(K_RESIZE_V, rv, 0, 0);
         push_launch(K_RESIZE_H, rh, 0, 0);
     }
@@ -889,6 +916,9 @@ int Context::run(const Batch& B, RunOutputs* outs) {
                 case K_LUMA: launch_luma((const LumaJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
                 case K_RESIZE_V: launch_resize_v((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
                 case K_RESIZE_H: launch_resize_h((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
+                case K_RESIZE_FUSED:
+                    launch_resize_fused((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st);
+                    break;
             }
             prof_end(L.kind, L.bytes, ev);
         }

@@ -730,7 +730,6 @@ __global__ __launch_bounds__(kThreads) void k_resize_v(const ResizeJob* __restri
 }
 
 constexpr int kResizeSpan = 2 * kThreads + 64;  // input floats for 256 outputs (ratio ~2)
-constexpr int kResizeRows = 8;                   // output rows per workgroup (share weights)
 
 __global__ __launch_bounds__(kThreads) void k_resize_h(const ResizeJob* __restrict__ jobs,
                                                        int njobs) {
@@ -763,6 +762,98 @@ __global__ __launch_bounds__(kThreads) void k_resize_h(const ResizeJob* __restri
     }
     for (int r = 0; r < nrows; r++) {
         const float* in = staged ? &span[r][l - lo] : J.tmp + (size_t)(y0 + r) * J.tmp_pitch + l;
+        float s = 0.0f;
+        if (n <= 16) {
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                if (i < n) s = mul_add_exact(s, in[i], wv[i]);
+        } else {
+            for (int i = 0; i < n; i++) s = mul_add_exact(s, in[i], wr[i]);
+        }
+        float c = s < 0.0f ? 0.0f : (s > 255.0f ? 255.0f : s);
+        J.dst[(size_t)(y0 + r) * J.dst_pitch + ox] = (uint8_t)roundf(c);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused Lanczos3 resize: one work-group computes kResizeRows output rows x 256
+// output columns. The vertical pass for those rows over the input columns the
+// tile's horizontal windows need, [lo4, hi) with lo4 = left[ox0] rounded down to
+// a multiple of 4, goes to LDS as f32 (the same t += p * w in window order as
+// k_resize_v: wave w takes output rows w and w + 4, so its weights are
+// wave-uniform, and lane l the column quads l, l + 64, ...); the horizontal pass
+// then reads it from LDS as k_resize_h does. The f32 intermediate never goes to
+// HBM. The host takes this kernel only when every tile's span fits
+// kResizeFusedSpan (octave steps, ratio ~2, need about 524 columns).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_resize_fused(const ResizeJob* __restrict__ jobs,
+                                                           int njobs) {
+    __shared__ __align__(16) float vt[kResizeRows][kResizeFusedSpan];
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const ResizeJob& J = jobs[find_job(jobs, njobs, tile)];
+    const int t = tile - J.tile_begin;
+    const int y0 = (t / J.tiles_x) * kResizeRows;
+    const int ox0 = (t % J.tiles_x) * kThreads;
+    const int oxl = min(ox0 + kThreads, J.out_w) - 1;
+    const int nrows = min(kResizeRows, J.out_h - y0);
+    const int lo4 = J.left[ox0] & ~3;
+    const int hi = J.left[oxl] + J.count[oxl];  // exclusive, <= in_w
+    const int nq = (hi - lo4 + 3) >> 2;         // column quads: 4 * nq <= kResizeFusedSpan
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int in_w = J.in_w, pitch = J.src_pitch;
+    for (int r = wave; r < nrows; r += kThreads / 64) {
+        const int oy = y0 + r;
+        const int l = J.vleft[oy], n = J.vcount[oy];
+        const ctap_t* wr = as_const_taps(J.vweights) + (size_t)oy * J.vmaxtaps;
+        for (int q = lane; q < nq; q += 64) {
+            const int x = lo4 + 4 * q;
+            const uint8_t* src = J.src_u8 + (size_t)l * pitch + x;
+            float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (x + 4 <= in_w) {
+                int i = 0;
+                for (; i + 4 <= n; i += 4) {
+                    unsigned u[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) u[k] = *(const unsigned*)(src + (size_t)(i + k) * pitch);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const float wk = wr[i + k];
+                        float p[4];
+                        cvt4(u[k], p);
+#pragma unroll
+                        for (int c = 0; c < 4; c++) s[c] = mul_add_exact(s[c], p[c], wk);
+                    }
+                }
+                for (; i < n; i++) {
+                    const float wk = wr[i];
+                    float p[4];
+                    cvt4(*(const unsigned*)(src + (size_t)i * pitch), p);
+#pragma unroll
+                    for (int c = 0; c < 4; c++) s[c] = mul_add_exact(s[c], p[c], wk);
+                }
+            } else {  // the quad at the right edge: columns >= in_w are never read back
+                for (int i = 0; i < n; i++) {
+                    const float wk = wr[i];
+#pragma unroll
+                    for (int c = 0; c < 4; c++)
+                        if (x + c < in_w) s[c] = mul_add_exact(s[c], (float)src[(size_t)i * pitch + c], wk);
+                }
+            }
+            *(float4*)&vt[r][4 * q] = make_float4(s[0], s[1], s[2], s[3]);
+        }
+    }
+    __syncthreads();
+    const int ox = ox0 + threadIdx.x;
+    if (ox >= J.out_w) return;
+    const int l = J.left[ox], n = J.count[ox];
+    const float* wr = J.weights + (size_t)ox * J.maxtaps;
+    float wv[16];  // this column's weights, loaded once for all rows
+    if (n <= 16) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) wv[i] = i < n ? wr[i] : 0.0f;
+    }
+    for (int r = 0; r < nrows; r++) {
+        const float* in = &vt[r][l - lo4];
         float s = 0.0f;
         if (n <= 16) {
 #pragma unroll
@@ -841,6 +932,11 @@ void launch_luma(const LumaJob* d_jobs, int njobs, int total_tiles, hipStream_t 
 void launch_resize_v(const ResizeJob* d_jobs, int njobs, int total_tiles, hipStream_t s) {
     if (total_tiles <= 0) return;
     hipLaunchKernelGGL(k_resize_v, dim3(total_tiles), dim3(kThreads), 0, s, d_jobs, njobs);
+}
+
+void launch_resize_fused(const ResizeJob* d_jobs, int njobs, int total_tiles, hipStream_t s) {
+    if (total_tiles <= 0) return;
+    hipLaunchKernelGGL(k_resize_fused, dim3(total_tiles), dim3(kThreads), 0, s, d_jobs, njobs);
 }
 
 void launch_resize_h(const ResizeJob* d_jobs, int njobs, int total_tiles, hipStream_t s) {

@@ -46,6 +46,7 @@ struct ParamTables {
 struct LanczosRef {
     int maxtaps = 0;
     size_t left_off = 0, count_off = 0, w_off = 0;
+    std::vector<int> left, count;  // host copies (tile spans of the fused resize)
 };
 
 struct ParamKey {

@@ -49,6 +49,11 @@ struct ResizeJob {
     const int* count;
     const float* weights;   // [out][maxtaps]
     int maxtaps;
+    // the vertical table, for the fused kernel (its left/count/weights are horizontal)
+    const int* vleft;
+    const int* vcount;
+    const float* vweights;
+    int vmaxtaps;
     int in_w, in_h, out_w, out_h;
     int src_pitch, tmp_pitch, dst_pitch;
     int tiles_x;
@@ -114,6 +119,10 @@ void launch_blur_h(const BlurJob* d_jobs, int njobs, int total_tiles, int max_pa
 void launch_blur_v(const BlurJob* d_jobs, int njobs, int total_tiles, int narrow_rows,
                    hipStream_t s);
 void launch_resize_v(const ResizeJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
+// Both Lanczos3 passes in one kernel, the f32 intermediate kept in LDS: for jobs
+// whose every tile of kResizeRows x 256 outputs needs at most kResizeFusedSpan
+// input columns (from left[ox0] rounded down to a multiple of 4).
+void launch_resize_fused(const ResizeJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
 void launch_resize_h(const ResizeJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
 void launch_copy_level(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
 // Small levels: one output per lane (tiles of 64 x 4). max_taps4 = the largest
@@ -167,6 +176,10 @@ constexpr int kMaxHPad = 3576;
 constexpr int kSmallMaxTaps4 = 2800;
 constexpr int kSmallMaxRows = 128;
 constexpr int kScanTileElems = 4096;
+// Lanczos3 resize tiles: output rows per work-group of the horizontal and fused
+// kernels, and the input columns the fused kernel stages per tile.
+constexpr int kResizeRows = 8;
+constexpr int kResizeFusedSpan = 576;
 // Rows of one extremum-test strip (64 columns wide, tested as bands of 16 rows):
 // kExtStripRows when a launch has at least kExtFillBands bands of 16 rows (enough
 // strips to fill the GPU several times over), else one band per strip.
