@@ -222,6 +222,14 @@ bool graphs_enabled_by_env() {
 }
 }  // namespace
 
+// The fused Lanczos3 octave step is used wherever its LDS span fits;
+// SIFT_HIP_RESIZE_FUSED=0 (read on every call) selects the two-pass form, so
+// that tests can compare the two.
+static bool resize_fusion_enabled() {
+    const char* e = getenv("SIFT_HIP_RESIZE_FUSED");
+    return !(e && e[0] == '0');
+}
+
 // Initial candidate capacity max(min, pixel-layers / divisor): 4096 and 128 by
 // default; SIFT_HIP_CAP_MIN / SIFT_HIP_CAP_DIVISOR override them (read on every
 // call) so that tests can force the overflow-and-retry path.
@@ -756,6 +764,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         }
         // Lanczos3 to the next octave base (src/sift.rs:166-173)
         std::vector<ResizeJob> rv, rh, rf;
+        const bool fuse_resize = resize_fusion_enabled();
         for (int i = 0; i < n; i++) {
             if ((int)shapes[i].size() <= o + 1) continue;
             const OctShape& a = shapes[i][o];
@@ -776,7 +785,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             J.tmp_pitch = a.pitch;
             J.dst_pitch = b.pitch;
             // both passes in one kernel when every tile's input span fits its LDS
-            bool fuse = true;
+            bool fuse = fuse_resize;
             for (int ox0 = 0; ox0 < b.w && fuse; ox0 += 256) {
                 const int oxl = std::min(ox0 + 256, b.w) - 1;
                 const int lo4 = LH.left[ox0] & ~3;

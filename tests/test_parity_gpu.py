@@ -43,6 +43,19 @@ def test_pyramid_and_dog_bit_exact(case):
         assert np.array_equal(d_hip[o], d_ref[o]), f"dog octave {o}"
 
 
+@pytest.mark.parametrize("case", ["1920x1080", "odd_97x61"])
+def test_two_pass_resize_equals_fused(case, monkeypatch):
+    """The two-pass Lanczos3 octave step (f32 rows through HBM) and the fused one
+    (rows in LDS) build the same pyramid."""
+    img, octv, ct = CASES[case]()
+    s, _ = _sift(octv, ct)
+    g_fused, _ = s.pyramid(img)
+    monkeypatch.setenv("SIFT_HIP_RESIZE_FUSED", "0")
+    g_two, _ = s.pyramid(img)
+    for o in range(octv):
+        assert np.array_equal(g_fused[o], g_two[o]), f"gauss octave {o}"
+
+
 @pytest.mark.parametrize("case", ["512x512", "1920x1080", "3840x2160", "odd_97x61"])
 def test_candidates_and_refined_bit_exact(case):
     img, octv, ct = CASES[case]()
