@@ -616,7 +616,8 @@ __global__ __launch_bounds__(kThreads) void k_refine(const OctaveDesc* __restric
 // k_orient: one wave (workgroup of 64) per refined keypoint, grid-stride
 // (src/sift.rs:605-726). For R <= 16 the window's rows and columns of the
 // Gaussian level (every pixel the gradient of a sample can touch) are staged
-// in LDS as q = v/255 floats. Lane l takes one contiguous chunk of the
+// in LDS as bytes; a sample forms q = v/255 of its 4 gradient texels with the
+// exact f32 q255f. Lane l takes one contiguous chunk of the
 // (2R+1)^2 window in raster order; each sample adds mag * exp(-(dx^2+dy^2)/denom)
 // to bin floor(theta * 36 / 2pi) of a u64 fixed-point (2^40) histogram, one of 4
 // copies by lane, with runs of same-bin samples summed in registers first
@@ -627,7 +628,7 @@ __global__ __launch_bounds__(kThreads) void k_refine(const OctaveDesc* __restric
 // ---------------------------------------------------------------------------
 constexpr int kOriMaxR = 16;
 constexpr int kOriRows = 2 * kOriMaxR + 5;  // staged rows
-constexpr int kOS = 2 * kOriMaxR + 8;       // staged row stride (floats): 4 * ceil((3 + rows + 3) / 4)
+constexpr int kOS = 2 * kOriMaxR + 8;       // staged row stride (bytes): 4 * ceil((3 + rows + 3) / 4)
 
 __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oct,
                                                const ImageParams* __restrict__ params,
@@ -637,10 +638,11 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                                                const int* __restrict__ valid,
                                                unsigned* __restrict__ npeaks,
                                                float* __restrict__ angles) {
-    // 4 histogram copies (lane & 3) and the staged window: 7 KB of LDS, so 22
-    // work-groups (one wave each) fit on a CU
+    // 4 histogram copies (lane & 3) and the staged window: 2.9 KB of LDS, so
+    // the 8 waves per SIMD that 40 VGPRs allow fit (32 work-groups of one wave
+    // per CU; 21 with the window staged as f32)
     __shared__ unsigned long long hist[4][36];
-    __shared__ __align__(16) float pq[kOriRows * kOS];
+    __shared__ __align__(16) uint8_t pq[kOriRows * kOS];
     __shared__ int ocx[2 * kOriMaxR + 1], ocy[2 * kOriMaxR + 1];
     const int lane = threadIdx.x;
     const int n = (int)min(*total, (unsigned)cap);
@@ -694,9 +696,7 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                             if (xw + b >= 0 && xw + b < w) u |= (unsigned)row[xw + b] << (8 * b);
                     }
                 }
-                *reinterpret_cast<float4*>(&pq[v * kOS + 4 * wd]) =
-                    make_float4(q255f((float)(u & 255u)), q255f((float)((u >> 8) & 255u)),
-                                q255f((float)((u >> 16) & 255u)), q255f((float)(u >> 24)));
+                *reinterpret_cast<unsigned*>(&pq[v * kOS + 4 * wd]) = u;
             }
             // per column and row of the window: the sample pixel's offset in the
             // staged window, or -1 when it is outside [1, w-2] x [1, h-2]
@@ -726,9 +726,9 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                 const int ox = ocx[dx + R], oy = ocy[dy + R];
                 in = (ox | oy) >= 0;
                 if (in) {
-                    const float* pp = &pq[oy + ox];
-                    gx = pp[1] - pp[-1];
-                    gy = pp[kOS] - pp[-kOS];
+                    const uint8_t* pp = &pq[oy + ox];
+                    gx = q255f((float)pp[1]) - q255f((float)pp[-1]);
+                    gy = q255f((float)pp[kOS]) - q255f((float)pp[-kOS]);
                 }
             } else {
                 const int yi = f2i_sat(roundf(yo + (float)dy));
