@@ -107,6 +107,21 @@ def test_parameter_variants(variant):
     assert m["desc_l2_max"] <= DESC_L2_TOL
 
 
+@pytest.mark.parametrize("sigma,S", [(1.9, 3), (1.6, 2)])
+def test_pyramid_all_tap_paths(sigma, S):
+    """Levels bit-exact on a frame wide enough for the 8-pixel horizontal pass at
+    octave 0, with scale schedules whose kernels have 13 taps (sigma 1.9: 9, 7,
+    9, 11, 13, 17 taps) and 17 or 21 taps (the block form; S = 2: 7, 9, 11, 15, 21
+    taps), so every unrolled tap count, every word phase of the staged rows and
+    the block form of both passes run."""
+    img = imgen.generic_image(640, 480, seed=41)
+    s, p = _sift(octaves=4, sigma=sigma, S=S)
+    g_hip, _ = s.pyramid(img)
+    g_ref, _ = oracle.build_pyramid(p, img)
+    for o, (a, b) in enumerate(zip(g_hip, g_ref)):
+        assert np.array_equal(a, b), f"gauss octave {o}"
+
+
 def test_uniform_image_has_no_keypoints():
     # tests/integration_test.rs:205-226
     s, _ = _sift()
