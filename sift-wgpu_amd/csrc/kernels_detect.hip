@@ -1,5 +1,5 @@
 // kernels_detect.hip -- gfx950 kernels for everything after the pyramid:
-//   k_extrema     find_scale_space_extrema      src/sift.rs:510-603
+//   k_extrema_tile / k_extrema  find_scale_space_extrema  src/sift.rs:510-603
 //   k_scan_*      ordered compaction offsets (exclusive scan)
 //   k_compact     candidate list in the reference order (octave, layer, row, column)
 //   k_refine      refine_and_filter_extrema     src/sift.rs:299-474 (+ solve 478-503)
@@ -9,7 +9,8 @@
 //
 // DoG values are never stored: every DoG sample is recomputed from the u8
 // Gaussian levels as q[G_{s+1}] - q[G_s] with q[v] = v / 255 (IEEE division,
-// kept in an LDS table), which is bit-identical to the reference's
+// formed without it by q255f, or read from an LDS table of it in the row-wise
+// extremum and wide-window kernels), which is bit-identical to the reference's
 // (G_{s+1} as f32 / 255.0) - (G_s as f32 / 255.0) (src/sift.rs:97,120).
 #include <algorithm>
 

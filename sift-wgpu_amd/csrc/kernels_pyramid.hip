@@ -2,7 +2,9 @@
 //   k_blur_h / k_blur_v : one separable pass of imageproc gaussian_blur_f32 on u8
 //                         (called at src/sift.rs:159,1211), optional DoG epilogue
 //                         (src/sift.rs:92-124,182-204) in the vertical pass.
-//   k_resize_v/_h       : image resize_exact(.., Lanczos3) (src/sift.rs:83-88).
+//   k_resize_fused      : image resize_exact(.., Lanczos3) (src/sift.rs:83-88), both
+//                         passes in one kernel; k_resize_v/_h: the two-pass form.
+//   k_luma              : colour -> grey (image 0.25.6 to_luma8, the wasm canvas rule).
 //
 // Exactness: every pass computes, per output pixel, acc = 0; acc = acc + p_i * w_i
 // in tap order with separate multiply and add (no FMA; the file is compiled with
