@@ -551,7 +551,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     const int cap = (int)cand_cap, kcap = (int)kp_cap;
 
     const size_t bits_off = C.take((size_t)std::max<long long>(nbits, 1) * 8);
-    const size_t counts_off = C.take((size_t)(nslots + 1) * 4);
+    const size_t counts_off = C.take((size_t)(nslots + 2) * 4);  // + the descriptor's list counts
     const size_t offs_off = C.take((size_t)(nslots + 2) * 4);
     const size_t scantmp1_off = C.take((size_t)(nslots / kScanTileElems + 4) * 4);
     const size_t cand_off = C.take((size_t)cap * sizeof(Cand));
@@ -566,6 +566,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     const size_t desc_off = C.take((size_t)kcap * 128 * 4);
     const size_t summary_off = C.take((size_t)(n + 3) * 4);
     const size_t wide_off = C.take((size_t)cap * 4);  // keypoints for the wide-window descriptor
+    const size_t mid_off = C.take((size_t)cap * 4);   // keypoints for the R <= 31 descriptor
     const size_t descu8_off = B.desc_u8 ? C.take((size_t)kcap * 128) : 0;
 
     // --- job blob (host), uploaded to the end of the arena ---
@@ -966,7 +967,8 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         prof_end(K_SCAN, 0.0, ev);
         prof_begin(K_DESCRIPTOR, 0.0, &ev);
         launch_descriptor(d_oct, d_par, d_cand, d_total, cap, d_ref, d_npk, d_ang, d_koff, kcap, d_kps,
-                          d_desc, d_counts + nslots, (int*)AP(wide_off), st);
+                          d_desc, d_counts + nslots, (int*)AP(wide_off), d_counts + nslots + 1,
+                          (int*)AP(mid_off), st);
         prof_end(K_DESCRIPTOR, 0.0, ev);
         prof_begin(K_FINALIZE, 0.0, &ev);
         launch_finalize(d_offs, (const int*)(D + isb_blob), n, d_koff, d_total, cap, d_sum, st);
@@ -993,7 +995,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
                                      oct_blob, par_blob, isb_blob, bits_off, counts_off, offs_off,
                                      scantmp1_off, cand_off, candresp_off, ref_off, valid_off, npk_off,
                                      ang_off, koff_off, scantmp2_off, kps_off, desc_off, summary_off,
-                                     wide_off, (uint64_t)B.desc_u8, descu8_off};
+                                     wide_off, mid_off, (uint64_t)B.desc_u8, descu8_off};
         for (const LaunchRec& L : launches) {
             key.push_back((uint64_t)L.kind << 32 | (uint32_t)L.njobs);
             key.push_back(L.job_off);

@@ -859,10 +859,18 @@ __device__ __forceinline__ float wave_sum_dpp(float v) {
     return __int_as_float(__builtin_amdgcn_readlane(x, 63));
 }
 
-constexpr int kDescMaxR = 31;                   // LDS path for sigma_octave <= 3.65
-constexpr int kGS = 2 * kDescMaxR + 3;          // bilinear grid row stride (floats)
-constexpr int kPatchRows = 2 * kDescMaxR + 6;   // staged patch rows
-constexpr int kPS = 72;                         // staged patch row stride (bytes, >= rows + 3)
+constexpr int kDescMaxR = 31;    // LDS path for sigma_octave <= 3.65
+// The instantiation for windows up to R = 28 (99.5% of the keypoints of the
+// default schedule, 22.6 KB of LDS) fits 7 work-groups per CU, the R = 31 one 6.
+constexpr int kDescSmallR = 28;
+// LDS sizes of the descriptor instantiation for windows up to MR.
+template <int MR>
+struct DescDims {
+    static constexpr int GS = 2 * MR + 3;         // bilinear grid row stride (floats)
+    static constexpr int PatchRows = 2 * MR + 6;  // staged patch rows
+    // staged patch row stride (bytes): the aligned words of a row, up to 3 + rows + 3 bytes
+    static constexpr int PS = (2 * MR + 12) / 4 * 4;
+};
 constexpr int kCellWords = 10;                  // one histogram cell: 8 angle bins + pair padding
 constexpr int kTableWords = 36 * kCellWords;    // 6 x 6 padded cells
 constexpr int kHistWords = 2 * kTableWords;     // one copy: even-pair table + odd-pair table
@@ -1045,7 +1053,7 @@ __device__ __forceinline__ void desc_finish(const unsigned* hist, float fix, int
 }
 
 // ---------------------------------------------------------------------------
-// k_descriptor: one workgroup (256) per refined keypoint with R <= 31
+// k_descriptor<MR>: one workgroup (256) per refined keypoint with R <= MR
 // (sigma_octave <= 3.65), looping over its orientation peaks
 // (src/sift.rs:917-1060).
 // Per keypoint: the clamped u8 patch of the Gaussian level around (x_o, y_o) --
@@ -1070,18 +1078,23 @@ __device__ __forceinline__ void desc_finish(const unsigned* hist, float fix, int
 // LDS adds; integer adds: independent of the order). Wave 0 sums the copies
 // and normalises.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_descriptor(
+template <int MR, bool kFromList>
+__global__ __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 7 : 6))) void k_descriptor(
     const OctaveDesc* __restrict__ oct, const ImageParams* __restrict__ params,
     const Cand* __restrict__ cand, const unsigned* __restrict__ total, int cap,
     const Keypoint* __restrict__ ref, const unsigned* __restrict__ npeaks,
     const float* __restrict__ angles, const unsigned* __restrict__ koff, int kcap,
-    Keypoint* __restrict__ kps_out, float* __restrict__ desc_out, unsigned* __restrict__ wide_count,
-    int* __restrict__ wide_list) {
+    Keypoint* __restrict__ kps_out, float* __restrict__ desc_out,
+    const unsigned* __restrict__ in_count, const int* __restrict__ in_list,
+    unsigned* __restrict__ out_count, int* __restrict__ out_list) {
+    using D = DescDims<MR>;
+    constexpr int kGS = D::GS, kPatchRows = D::PatchRows, kPS = D::PS;
     static_assert(kHistCopies * kHistWords * 4 >= kPatchRows * kPS + 4, "patch shares the histograms");
     // the u8 patch is dead once the grid is built; the histograms live after that
     __shared__ __align__(16) unsigned pu[kHistCopies * kHistWords];
     __shared__ float grid[kGS * kGS];
-    __shared__ float exs[2 * kDescMaxR + 1];
+    __shared__ float exs[2 * MR + 1];
     __shared__ float4 rrec[64];  // per row: first and last column (int bits), sin * py, cos * py
     __shared__ float rwy[64];
     __shared__ int rpre[65];
@@ -1091,12 +1104,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
     uint8_t* const patch = reinterpret_cast<uint8_t*>(pu);
     unsigned* const hist = pu;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int n = (int)min(*total, (unsigned)cap);
-    for (int c = blockIdx.x; c < n; c += gridDim.x) {
+    // all candidates, or the ones an instantiation for smaller windows listed
+    const int n = (int)min(kFromList ? *in_count : *total, (unsigned)cap);
+    for (int idx = blockIdx.x; idx < n; idx += gridDim.x) {
+        const int c = kFromList ? in_list[idx] : idx;
         const DescGeom G = desc_geom(c, oct, params, cand, ref, npeaks, koff);
         if (G.np == 0) continue;
-        if (G.R > kDescMaxR) {  // wider windows: listed for k_descriptor_wide
-            if (threadIdx.x == 0) wide_list[atomicAdd(wide_count, 1u)] = c;
+        if (G.R > MR) {  // wider windows: listed for the next instantiation
+            if (threadIdx.x == 0) out_list[atomicAdd(out_count, 1u)] = c;
             continue;
         }
         const int R = G.R, side = G.side, gside = 2 * R + 3;
@@ -1400,8 +1415,8 @@ __global__ __launch_bounds__(256) void k_desc_u8(const float4* __restrict__ desc
 
 void launch_extrema(const OctaveDesc* d_oct, int noct, int total_rows, const ImageParams* d_params,
                     unsigned long long* d_bits, unsigned* d_counts, int nslots, hipStream_t s) {
-    // counts[nslots] is the descriptor's wide-window keypoint count
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, d_counts + nslots, 1, 0u);
+    // counts[nslots], counts[nslots + 1]: the descriptor's list counts
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, d_counts + nslots, 2, 0u);
     if (total_rows <= 0) return;
     hipLaunchKernelGGL(k_extrema, dim3(total_rows), dim3(kThreads), 0, s, d_oct, noct, d_params,
                        d_bits, d_counts);
@@ -1412,8 +1427,8 @@ bool extrema_tile_supported(int S) { return S >= 1 && S <= 6; }
 void launch_extrema_tile(int S, int strip_rows, const OctaveDesc* d_oct, int noct, int total_tiles,
                          const ImageParams* d_params, unsigned long long* d_bits, unsigned* d_counts,
                          int nslots, hipStream_t s) {
-    // counts[nslots] is the descriptor's wide-window keypoint count
-    hipLaunchKernelGGL(k_fill_u32, dim3((nslots + 1 + 255) / 256), dim3(256), 0, s, d_counts, nslots + 1, 0u);
+    // counts[nslots], counts[nslots + 1]: the descriptor's list counts
+    hipLaunchKernelGGL(k_fill_u32, dim3((nslots + 2 + 255) / 256), dim3(256), 0, s, d_counts, nslots + 2, 0u);
     if (total_tiles <= 0) return;
     switch (S) {
 #define SH_EXT(SS) \
@@ -1470,11 +1485,18 @@ void launch_descriptor(const OctaveDesc* d_oct, const ImageParams* d_params, con
                        const unsigned* d_total, int cap, const Keypoint* d_ref,
                        const unsigned* d_npeaks, const float* d_angles, const unsigned* d_koff,
                        int kcap, Keypoint* d_kps, float* d_desc, unsigned* d_wide_count,
-                       int* d_wide_list, hipStream_t s) {
-    // as k_orient: few keypoints per workgroup, so the launch ends evenly
-    hipLaunchKernelGGL(k_descriptor, dim3(grid_for(cap, 1, 49152)), dim3(kThreads), 0, s, d_oct,
+                       int* d_wide_list, unsigned* d_mid_count, int* d_mid_list, hipStream_t s) {
+    // as k_orient: few keypoints per workgroup, so the launch ends evenly. Windows up
+    // to kDescSmallR run at 7 work-groups per CU and list the few others for the
+    // kDescMaxR instantiation (6 per CU), which lists still wider ones for
+    // k_descriptor_wide.
+    const int grid = grid_for(cap, 1, 49152);
+    hipLaunchKernelGGL((k_descriptor<kDescSmallR, false>), dim3(grid), dim3(kThreads), 0, s, d_oct,
                        d_params, d_cand, d_total, cap, d_ref, d_npeaks, d_angles, d_koff, kcap,
-                       d_kps, d_desc, d_wide_count, d_wide_list);
+                       d_kps, d_desc, nullptr, nullptr, d_mid_count, d_mid_list);
+    hipLaunchKernelGGL((k_descriptor<kDescMaxR, true>), dim3(grid), dim3(kThreads), 0, s, d_oct,
+                       d_params, d_cand, d_total, cap, d_ref, d_npeaks, d_angles, d_koff, kcap,
+                       d_kps, d_desc, d_mid_count, d_mid_list, d_wide_count, d_wide_list);
     hipLaunchKernelGGL(k_descriptor_wide, dim3(256), dim3(kThreads), 0, s, d_oct, d_params, d_cand,
                        d_total, cap, d_ref, d_npeaks, d_angles, d_koff, kcap, d_kps, d_desc,
                        d_wide_count, d_wide_list);

@@ -160,7 +160,7 @@ void launch_descriptor(const OctaveDesc* d_oct, const ImageParams* d_params, con
                        const unsigned* d_total, int cap, const Keypoint* d_ref,
                        const unsigned* d_npeaks, const float* d_angles, const unsigned* d_koff,
                        int kcap, Keypoint* d_kps, float* d_desc, unsigned* d_wide_count,
-                       int* d_wide_list, hipStream_t s);
+                       int* d_wide_list, unsigned* d_mid_count, int* d_mid_list, hipStream_t s);
 // Writes summary[0] = candidates, summary[1] = keypoints, summary[2..2+nimg] =
 // per-image keypoint offsets (nimg+1 entries).
 void launch_finalize(const unsigned* d_slot_offsets, const int* d_img_slot_begin, int nimg,
