@@ -456,52 +456,94 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(const unsigned* __restri
 }
 
 // ---------------------------------------------------------------------------
-// k_compact: one wave per (octave, layer, row) slot. Reads the row's bitmask
-// words, ranks the set bits with popcounts and a wave prefix sum and writes the
-// candidates at offsets[slot] + rank, i.e. in the reference's order.
+// k_compact: one wave per spw consecutive (octave, layer, row) slots (spw <=
+// kCompactSlots: 8 for launches with many slots, 1 for single frames, whose
+// few slots would otherwise leave most of the GPU idle).
+// The slots' counts and offsets are read once (one lane each) and the octave is
+// found once; the bitmask words of every slot with candidates (one lane per
+// word, rows of at most 64 words) are loaded before any slot is written, so
+// their loads overlap. For each slot the set bits are ranked with popcounts and
+// a wave prefix sum and the candidates written at offsets[slot] + rank, i.e. in
+// the reference's order. Rows of more than 64 words take a chunked loop.
 // ---------------------------------------------------------------------------
+constexpr int kCompactSlots = 8;
+
+__device__ __forceinline__ void compact_word(unsigned long long m, int wi, int j, int y, int s,
+                                             unsigned& base, int lane, Cand* __restrict__ cand,
+                                             int cap) {
+    const unsigned c = (unsigned)__popcll(m);
+    unsigned x = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned t = __shfl_up(x, d, 64);
+        if (lane >= d) x += t;
+    }
+    unsigned pre = base + x - c;
+    const unsigned tot = __shfl(x, 63, 64);
+    while (m) {
+        const int b = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        if (pre < (unsigned)cap) {
+            Cand cd;
+            cd.job = j;
+            cd.x = wi * 64 + b;
+            cd.y = y;
+            cd.layer = s;
+            cand[pre] = cd;
+        }
+        pre++;
+    }
+    base += tot;
+}
+
 __global__ __launch_bounds__(kThreads) void k_compact(const OctaveDesc* __restrict__ oct, int noct,
                                                       int nslots,
                                                       const unsigned long long* __restrict__ bits,
                                                       const unsigned* __restrict__ counts,
                                                       const unsigned* __restrict__ offsets,
-                                                      Cand* __restrict__ cand, int cap) {
+                                                      Cand* __restrict__ cand, int cap, int spw) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int slot = blockIdx.x * (kThreads / 64) + wave;
-    if (slot >= nslots) return;
-    if (counts[slot] == 0) return;
-    const int j = find_by(oct, noct, slot, &OctaveDesc::slot_begin);
-    const OctaveDesc& O = oct[j];
-    const int local = slot - O.slot_begin;
-    const int s = local / O.h + 1, y = local - (s - 1) * O.h;
-    const unsigned long long* wrow = bits + O.bits_begin + (long long)local * O.nwords;
-    unsigned base = offsets[slot];
-    for (int w0 = 0; w0 < O.nwords; w0 += 64) {
-        const int wi = w0 + lane;
-        unsigned long long m = wi < O.nwords ? wrow[wi] : 0ull;
-        unsigned c = (unsigned)__popcll(m);
-        unsigned x = c;
+    const int s0 = (blockIdx.x * (kThreads / 64) + wave) * spw;
+    if (s0 >= nslots) return;
+    const int ns = min(spw, nslots - s0);
+    const unsigned cnt_l = lane < ns ? counts[s0 + lane] : 0u;
+    const unsigned off_l = lane < ns ? offsets[s0 + lane] : 0u;
+    const unsigned long long any = ballot64(cnt_l != 0u);
+    if (any == 0ull) return;
+    // each slot's octave (the slots are ordered by octave)
+    int jk[kCompactSlots];
+    int j = find_by(oct, noct, s0, &OctaveDesc::slot_begin);
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            unsigned t = __shfl_up(x, d, 64);
-            if (lane >= d) x += t;
+    for (int k = 0; k < kCompactSlots; k++) {
+        while (k < ns && j + 1 < noct && oct[j + 1].slot_begin <= s0 + k) j++;
+        jk[k] = j;
+    }
+    // the bitmask words of the slots with candidates, all loads issued first
+    unsigned long long m[kCompactSlots];
+#pragma unroll
+    for (int k = 0; k < kCompactSlots; k++) {
+        m[k] = 0ull;
+        if ((any >> k) & 1ull) {
+            const OctaveDesc& O = oct[jk[k]];
+            if (lane < O.nwords)
+                m[k] = bits[O.bits_begin + (long long)(s0 + k - O.slot_begin) * O.nwords + lane];
         }
-        unsigned pre = base + x - c;
-        unsigned tot = __shfl(x, 63, 64);
-        while (m) {
-            int b = __ffsll((long long)m) - 1;
-            m &= m - 1;
-            if (pre < (unsigned)cap) {
-                Cand cd;
-                cd.job = j;
-                cd.x = wi * 64 + b;
-                cd.y = y;
-                cd.layer = s;
-                cand[pre] = cd;
+    }
+#pragma unroll
+    for (int k = 0; k < kCompactSlots; k++) {
+        if (!((any >> k) & 1ull)) continue;
+        const OctaveDesc& O = oct[jk[k]];
+        const int local = s0 + k - O.slot_begin;
+        const int s = local / O.h + 1, y = local - (s - 1) * O.h;
+        unsigned base = __shfl(off_l, k, 64);
+        compact_word(m[k], lane, jk[k], y, s, base, lane, cand, cap);
+        if (O.nwords > 64) {  // rows wider than 4096 px: the remaining words by chunks
+            const unsigned long long* wrow = bits + O.bits_begin + (long long)local * O.nwords;
+            for (int w0 = 64; w0 < O.nwords; w0 += 64) {
+                const int wi = w0 + lane;
+                compact_word(wi < O.nwords ? wrow[wi] : 0ull, wi, jk[k], y, s, base, lane, cand, cap);
             }
-            pre++;
         }
-        base += tot;
     }
 }
 
@@ -1455,9 +1497,10 @@ void launch_compact(const OctaveDesc* d_oct, int noct, int nslots, const unsigne
                     const unsigned* d_counts, const unsigned* d_offsets, Cand* d_cand, int cap,
                     hipStream_t s) {
     if (nslots <= 0) return;
-    const int per = kThreads / 64;
+    const int spw = nslots >= 65536 ? kCompactSlots : 1;  // slots per wave
+    const int per = (kThreads / 64) * spw;                 // slots per work-group
     hipLaunchKernelGGL(k_compact, dim3((nslots + per - 1) / per), dim3(kThreads), 0, s, d_oct, noct,
-                       nslots, d_bits, d_counts, d_offsets, d_cand, cap);
+                       nslots, d_bits, d_counts, d_offsets, d_cand, cap, spw);
 }
 
 static int grid_for(int cap, int per_block, int max_blocks) {
