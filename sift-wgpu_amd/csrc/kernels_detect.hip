@@ -659,8 +659,8 @@ __global__ __launch_bounds__(kThreads) void k_refine(const OctaveDesc* __restric
 // k_orient: one wave (workgroup of 64) per refined keypoint, grid-stride
 // (src/sift.rs:605-726). For R <= 16 the window's rows and columns of the
 // Gaussian level (every pixel the gradient of a sample can touch) are staged
-// in LDS as bytes; a sample forms q = v/255 of its 4 gradient texels with the
-// exact f32 q255f. Lane l takes one contiguous chunk of the
+// in LDS as bytes; a sample reads q = v/255 of its 4 gradient texels from an
+// LDS table filled with the exact f32 q255f. Lane l takes one contiguous chunk of the
 // (2R+1)^2 window in raster order; each sample adds mag * exp(-(dx^2+dy^2)/denom)
 // to bin floor(theta * 36 / 2pi) of a u64 fixed-point (2^40) histogram, one of 4
 // copies by lane, with runs of same-bin samples summed in registers first
@@ -681,10 +681,14 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                                                const int* __restrict__ valid,
                                                unsigned* __restrict__ npeaks,
                                                float* __restrict__ angles) {
-    // 4 histogram copies (lane & 3) and the staged window: 2.9 KB of LDS, so
-    // the 8 waves per SIMD that 40 VGPRs allow fit (32 work-groups of one wave
-    // per CU; 21 with the window staged as f32)
+    // 4 histogram copies (lane & 3), the staged window and the q table: 3.9 KB of
+    // LDS, so the 8 waves per SIMD that 40 VGPRs allow fit (32 work-groups of one
+    // wave per CU; 21 with the window staged as f32)
     __shared__ unsigned long long hist[4][36];
+    // v / 255 for every byte (q255f, exact): 4 LDS reads per sample instead of 16 VALU
+    // operations; written before the first keypoint's barrier, which orders it
+    __shared__ float qt[256];
+    for (int i = threadIdx.x; i < 256; i += 64) qt[i] = q255f((float)i);
     __shared__ __align__(16) uint8_t pq[kOriRows * kOS];
     __shared__ int ocx[2 * kOriMaxR + 1], ocy[2 * kOriMaxR + 1];
     const int lane = threadIdx.x;
@@ -770,8 +774,8 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                 in = (ox | oy) >= 0;
                 if (in) {
                     const uint8_t* pp = &pq[oy + ox];
-                    gx = q255f((float)pp[1]) - q255f((float)pp[-1]);
-                    gy = q255f((float)pp[kOS]) - q255f((float)pp[-kOS]);
+                    gx = qt[pp[1]] - qt[pp[-1]];
+                    gy = qt[pp[kOS]] - qt[pp[-kOS]];
                 }
             } else {
                 const int yi = f2i_sat(roundf(yo + (float)dy));
