@@ -29,6 +29,8 @@ CASES = {
     "3840x2160": lambda: imgen.config_image("3840x2160"),  # 3840x2160, 6 octaves, contrast 0.03
     "odd_97x61": lambda: (imgen.generic_image(97, 61, seed=9), 3, 0.04),
     "wide_333x45": lambda: (imgen.generic_image(333, 45, seed=10), 4, 0.04),
+    # rows of 65 bitmask words (wider than 4096 px): the compaction's chunked path
+    "wide_4160x200": lambda: (imgen.generic_image(4160, 200, seed=11), 4, 0.04),
 }
 
 
@@ -56,7 +58,7 @@ def test_two_pass_resize_equals_fused(case, monkeypatch):
         assert np.array_equal(g_fused[o], g_two[o]), f"gauss octave {o}"
 
 
-@pytest.mark.parametrize("case", ["512x512", "1920x1080", "3840x2160", "odd_97x61"])
+@pytest.mark.parametrize("case", ["512x512", "1920x1080", "3840x2160", "odd_97x61", "wide_4160x200"])
 def test_candidates_and_refined_bit_exact(case):
     img, octv, ct = CASES[case]()
     s, p = _sift(octv, ct)
