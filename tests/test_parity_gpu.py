@@ -73,6 +73,28 @@ def test_candidates_and_refined_bit_exact(case):
     assert np.allclose(ref["size"], oref["size"], rtol=2e-7, atol=0)
 
 
+@pytest.mark.parametrize("S", [1, 5, 6])
+def test_candidates_bit_exact_interval_counts(S):
+    """The extremum kernel is instantiated per interval count (S + 2 DoG layers in
+    LDS): candidates and refined keypoints bit-exact for the counts the other
+    cases do not use, on single frames and, through a batch large enough for the
+    multi-band strips, on 8 frames at once."""
+    img = imgen.generic_image(640, 480, seed=50 + S)
+    s, p = _sift(octaves=4, S=S)
+    cand, ref, _, _ = s.detect_stages(img)
+    _, _, ocand, oref, _ = oracle.detect_and_compute(p, img, stages=True)
+    assert len(cand) > 0
+    assert cand.tobytes() == ocand.tobytes()
+    for f in ("x", "y", "response", "octave", "layer"):
+        assert np.array_equal(ref[f], oref[f]), f
+    frames = [imgen.generic_image(1920, 1080, seed=60 + 10 * S + i) for i in range(8)]
+    together = s.detect_batch(frames, [s] * len(frames))
+    for i, f in enumerate(frames):
+        alone = s.detect_and_compute(f)
+        assert alone[0].tobytes() == together[i][0].tobytes(), i
+        assert alone[1].tobytes() == together[i][1].tobytes(), i
+
+
 def _parity(img, s, p):
     kps, desc = s.detect_and_compute(img)
     okps, odesc = oracle.detect_and_compute(p, img)
