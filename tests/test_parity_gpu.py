@@ -146,6 +146,18 @@ def test_pyramid_all_tap_paths(sigma, S):
         assert np.array_equal(a, b), f"gauss octave {o}"
 
 
+def test_wide_windows_match_oracle():
+    """sigma 2.0: 20% of the keypoints have descriptor windows wider than the LDS
+    forms take (R > 31) and go through k_descriptor_wide; others pass through the
+    R <= 28 and R <= 31 forms. All must match the oracle."""
+    img = imgen.generic_image(640, 480, seed=70)
+    s, p = _sift(octaves=4, sigma=2.0)
+    kps, desc, okps, m = _parity(img, s, p)
+    assert len(okps) > 0
+    assert m["precision"] >= MIN_PR and m["recall"] >= MIN_PR
+    assert m["desc_l2_max"] <= DESC_L2_TOL
+
+
 def test_uniform_image_has_no_keypoints():
     # tests/integration_test.rs:205-226
     s, _ = _sift()
