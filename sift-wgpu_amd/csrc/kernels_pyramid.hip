@@ -175,8 +175,8 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
     // word start at or before x0-r (clamped) as f32, and reads only that row, so
     // the waves need no work-group barrier. Aligned 4-byte words fully inside
     // [0, w) are read with one load; words at the image edges (and unaligned
-    // sources) fall back to clamped bytes. The first kHWordsUnrolled words of a
-    // lane are all loaded before any is written, so their loads overlap.
+    // sources) fall back to clamped bytes. The first KU words of a lane are all
+    // loaded before any is written, so their loads overlap.
     const int xs = x0 - r;
     const int ws = xs & ~3;                      // first word start (may be < 0)
     const int d = xs - ws;                       // element base + k is staged element base + d + k
