@@ -70,6 +70,17 @@ def lib():
         L.oracle_orientation_single.restype = ctypes.c_int
         L.oracle_descriptor_single.argtypes = [P(OracleParams), u8p, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_void_p, f32p]
+        L.oracle_orientation_histogram_single.argtypes = [P(OracleParams), u8p, ctypes.c_int,
+                                                          ctypes.c_int, ctypes.c_void_p, f32p]
+        L.oracle_orientation_histogram_single.restype = ctypes.c_int
+        L.oracle_smooth_orientation_histogram.argtypes = [f32p]
+        L.oracle_orientation_peaks.argtypes = [f32p, f32p]
+        L.oracle_orientation_peaks.restype = ctypes.c_int
+        L.oracle_descriptor_raw_single.argtypes = [P(OracleParams), u8p, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.c_void_p, f32p]
+        L.oracle_refine_single.argtypes = [P(OracleParams), f32p, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_refine_single.restype = ctypes.c_int
         L.oracle_to_luma8.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
                                       u8p]
         L.oracle_descriptor_u8.argtypes = [f32p, ctypes.c_size_t, u8p]
@@ -290,3 +301,55 @@ def descriptor_single(p, g, x, y, size, angle, layer=1):
     lib().oracle_descriptor_single(ctypes.byref(p), _u8(g), g.shape[1], g.shape[0],
                                    k.ctypes.data, _f32(out))
     return out
+
+
+def orientation_histogram_single(p, g, x, y, size, layer=1):
+    """The raw 36-bin orientation histogram of one keypoint (before smoothing)."""
+    g = np.ascontiguousarray(g, np.uint8)
+    k = _kp_struct(x, y, size, 0.0, layer)
+    out = np.zeros(36, np.float32)
+    lib().oracle_orientation_histogram_single(ctypes.byref(p), _u8(g), g.shape[1], g.shape[0],
+                                              k.ctypes.data, _f32(out))
+    return out
+
+
+def smooth_orientation_histogram(hist):
+    h = np.ascontiguousarray(hist, np.float32).copy()
+    assert h.size == 36
+    lib().oracle_smooth_orientation_histogram(_f32(h))
+    return h
+
+
+def orientation_peaks(hist):
+    h = np.ascontiguousarray(hist, np.float32)
+    out = np.zeros(36, np.float32)
+    n = lib().oracle_orientation_peaks(_f32(h), _f32(out))
+    return out[:n].copy()
+
+
+def descriptor_raw_single(p, g, x, y, size, angle, layer=1):
+    """The 4 x 4 x 8 descriptor histogram of one keypoint before normalisation,
+    indexed [(iy * 4 + ix) * 8 + ia]."""
+    g = np.ascontiguousarray(g, np.uint8)
+    k = _kp_struct(x, y, size, angle, layer)
+    out = np.zeros(128, np.float32)
+    lib().oracle_descriptor_raw_single(ctypes.byref(p), _u8(g), g.shape[1], g.shape[0],
+                                       k.ctypes.data, _f32(out))
+    return out
+
+
+def refine_single(p, dog, x, y, layer, octave=0):
+    """Refinement of one candidate at octave-pixel (x, y) -- placed as the
+    extremum search places it, ((x + 0.5) * 2^octave) -- on DoG layers
+    dog[nd, h, w]. Returns the refined keypoint or None."""
+    dog = np.ascontiguousarray(dog, np.float32)
+    nd, h, w = dog.shape
+    c = np.zeros(1, KEYPOINT_DTYPE)
+    sc = np.float32(2.0) ** octave
+    c["x"] = (np.float32(x) + np.float32(0.5)) * sc
+    c["y"] = (np.float32(y) + np.float32(0.5)) * sc
+    c["layer"], c["octave"] = layer, octave
+    out = np.zeros(1, KEYPOINT_DTYPE)
+    ok = lib().oracle_refine_single(ctypes.byref(p), _f32(dog), w, h, nd, c.ctypes.data,
+                                    out.ctypes.data)
+    return out[0] if ok else None

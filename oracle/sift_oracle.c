@@ -560,84 +560,105 @@ static void find_extrema(const oracle_params_t* p, const pyr_t* P, kpvec_t* out)
  * refine_and_filter_extrema (src/sift.rs:299-474); sequential over candidates
  * ------------------------------------------------------------------------ */
 
+/* One candidate: dl[0..nd) are the octave's DoG layers (w x h each). Returns 1
+ * and writes the refined keypoint when it passes, 0 when it is dropped. */
+static int refine_one(const oracle_params_t* p, const float* const* dl, int nd, int w, int h,
+                      const oracle_keypoint_t* kp, oracle_keypoint_t* outp) {
+    float xi = kp->x / pow2i(kp->octave); /* :309-310 */
+    float yi = kp->y / pow2i(kp->octave);
+    int cx = f2i_sat(xi), cy = f2i_sat(yi), cl = kp->layer;
+    for (int step = 0; step < 5; step++) { /* MAX_INTERPOLATION_STEPS */
+        if (cl < 1 || cl >= nd - 1) break; /* :323 */
+        const float* ip = dl[cl - 1];
+        const float* ic = dl[cl];
+        const float* in = dl[cl + 1];
+        if (cx < 1 || cx >= w - 1 || cy < 1 || cy >= h - 1) break; /* :330-336 */
+        float dx = (dog_px(ic, w, h, cx + 1, cy) - dog_px(ic, w, h, cx - 1, cy)) / 2.0f;
+        float dy = (dog_px(ic, w, h, cx, cy + 1) - dog_px(ic, w, h, cx, cy - 1)) / 2.0f;
+        float ds = (dog_px(in, w, h, cx, cy) - dog_px(ip, w, h, cx, cy)) / 2.0f;
+        float cv = dog_px(ic, w, h, cx, cy);
+        float dxx = dog_px(ic, w, h, cx + 1, cy) + dog_px(ic, w, h, cx - 1, cy) - 2.0f * cv;
+        float dyy = dog_px(ic, w, h, cx, cy + 1) + dog_px(ic, w, h, cx, cy - 1) - 2.0f * cv;
+        float dss = dog_px(in, w, h, cx, cy) + dog_px(ip, w, h, cx, cy) - 2.0f * cv;
+        float dxy = (dog_px(ic, w, h, cx + 1, cy + 1) - dog_px(ic, w, h, cx - 1, cy + 1) -
+                     dog_px(ic, w, h, cx + 1, cy - 1) + dog_px(ic, w, h, cx - 1, cy - 1)) /
+                    4.0f;
+        float dxs = (dog_px(in, w, h, cx + 1, cy) - dog_px(in, w, h, cx - 1, cy) -
+                     (dog_px(ip, w, h, cx + 1, cy) - dog_px(ip, w, h, cx - 1, cy))) /
+                    4.0f;
+        float dys = (dog_px(in, w, h, cx, cy + 1) - dog_px(in, w, h, cx, cy - 1) -
+                     (dog_px(ip, w, h, cx, cy + 1) - dog_px(ip, w, h, cx, cy - 1))) /
+                    4.0f;
+        float H[3][3] = {{dxx, dxy, dxs}, {dxy, dyy, dys}, {dxs, dys, dss}};
+        float b[3] = {-dx, -dy, -ds};
+        float off[3];
+        if (!solve3(H, b, off)) break; /* :460-463 */
+        if (fabsf(off[0]) < 0.5f && fabsf(off[1]) < 0.5f && fabsf(off[2]) < 0.5f) {
+            float val = cv + 0.5f * (dx * off[0] + dy * off[1] + ds * off[2]); /* :394-398 */
+            if (fabsf(val) < p->contrast_threshold) break;                     /* :401 */
+            float tr = dxx + dyy;
+            float trace_sq = tr * tr;
+            float det = dxx * dyy - dxy * dxy;
+            if (det <= 0.0f) break; /* :411 */
+            float ratio = trace_sq / det;
+            float e1 = p->edge_threshold + 1.0f;
+            float thr = e1 * e1 / p->edge_threshold;
+            if (ratio >= thr) break; /* :420 */
+            float scale = pow2i(kp->octave);
+            float fl = (float)cl + off[2];
+            float psig = p->sigma * powf(2.0f, (float)kp->octave + fl / (float)p->num_intervals);
+            oracle_keypoint_t r;
+            r.x = ((float)cx + off[0]) * scale;
+            r.y = ((float)cy + off[1]) * scale;
+            r.size = psig;
+            r.angle = 0.0f;
+            r.response = val;
+            r.octave = kp->octave;
+            r.layer = f2i_sat(roundf(fl));
+            *outp = r;
+            return 1;
+        } else {
+            cx = f2i_sat(roundf((float)cx + off[0])); /* :451-453 */
+            cy = f2i_sat(roundf((float)cy + off[1]));
+            cl = f2i_sat(roundf((float)cl + off[2]));
+            if (cl < 0 || cl >= nd) break; /* :456 */
+        }
+    }
+    return 0;
+}
+
 static void refine(const oracle_params_t* p, const pyr_t* P, const kpvec_t* cand, kpvec_t* out) {
     int nd = P->L - 1;
     for (size_t ci = 0; ci < cand->n; ci++) {
         const oracle_keypoint_t* kp = &cand->v[ci];
         int o = kp->octave;
-        float xi = kp->x / pow2i(kp->octave); /* :309-310 */
-        float yi = kp->y / pow2i(kp->octave);
-        int cx = f2i_sat(xi), cy = f2i_sat(yi), cl = kp->layer;
-        int w = P->w[o], h = P->h[o];
-        for (int step = 0; step < 5; step++) { /* MAX_INTERPOLATION_STEPS */
-            if (cl < 1 || cl >= nd - 1) break; /* :323 */
-            const float* ip = P->d[o * nd + cl - 1];
-            const float* ic = P->d[o * nd + cl];
-            const float* in = P->d[o * nd + cl + 1];
-            if (cx < 1 || cx >= w - 1 || cy < 1 || cy >= h - 1) break; /* :330-336 */
-            float dx = (dog_px(ic, w, h, cx + 1, cy) - dog_px(ic, w, h, cx - 1, cy)) / 2.0f;
-            float dy = (dog_px(ic, w, h, cx, cy + 1) - dog_px(ic, w, h, cx, cy - 1)) / 2.0f;
-            float ds = (dog_px(in, w, h, cx, cy) - dog_px(ip, w, h, cx, cy)) / 2.0f;
-            float cv = dog_px(ic, w, h, cx, cy);
-            float dxx = dog_px(ic, w, h, cx + 1, cy) + dog_px(ic, w, h, cx - 1, cy) - 2.0f * cv;
-            float dyy = dog_px(ic, w, h, cx, cy + 1) + dog_px(ic, w, h, cx, cy - 1) - 2.0f * cv;
-            float dss = dog_px(in, w, h, cx, cy) + dog_px(ip, w, h, cx, cy) - 2.0f * cv;
-            float dxy = (dog_px(ic, w, h, cx + 1, cy + 1) - dog_px(ic, w, h, cx - 1, cy + 1) -
-                         dog_px(ic, w, h, cx + 1, cy - 1) + dog_px(ic, w, h, cx - 1, cy - 1)) /
-                        4.0f;
-            float dxs = (dog_px(in, w, h, cx + 1, cy) - dog_px(in, w, h, cx - 1, cy) -
-                         (dog_px(ip, w, h, cx + 1, cy) - dog_px(ip, w, h, cx - 1, cy))) /
-                        4.0f;
-            float dys = (dog_px(in, w, h, cx, cy + 1) - dog_px(in, w, h, cx, cy - 1) -
-                         (dog_px(ip, w, h, cx, cy + 1) - dog_px(ip, w, h, cx, cy - 1))) /
-                        4.0f;
-            float H[3][3] = {{dxx, dxy, dxs}, {dxy, dyy, dys}, {dxs, dys, dss}};
-            float b[3] = {-dx, -dy, -ds};
-            float off[3];
-            if (!solve3(H, b, off)) break; /* :460-463 */
-            if (fabsf(off[0]) < 0.5f && fabsf(off[1]) < 0.5f && fabsf(off[2]) < 0.5f) {
-                float val = cv + 0.5f * (dx * off[0] + dy * off[1] + ds * off[2]); /* :394-398 */
-                if (fabsf(val) < p->contrast_threshold) break;                     /* :401 */
-                float tr = dxx + dyy;
-                float trace_sq = tr * tr;
-                float det = dxx * dyy - dxy * dxy;
-                if (det <= 0.0f) break; /* :411 */
-                float ratio = trace_sq / det;
-                float e1 = p->edge_threshold + 1.0f;
-                float thr = e1 * e1 / p->edge_threshold;
-                if (ratio >= thr) break; /* :420 */
-                float scale = pow2i(kp->octave);
-                float fl = (float)cl + off[2];
-                float psig = p->sigma * powf(2.0f, (float)kp->octave + fl / (float)p->num_intervals);
-                oracle_keypoint_t r;
-                r.x = ((float)cx + off[0]) * scale;
-                r.y = ((float)cy + off[1]) * scale;
-                r.size = psig;
-                r.angle = 0.0f;
-                r.response = val;
-                r.octave = kp->octave;
-                r.layer = f2i_sat(roundf(fl));
-                kv_push(out, r);
-                break;
-            } else {
-                cx = f2i_sat(roundf((float)cx + off[0])); /* :451-453 */
-                cy = f2i_sat(roundf((float)cy + off[1]));
-                cl = f2i_sat(roundf((float)cl + off[2]));
-                if (cl < 0 || cl >= nd) break; /* :456 */
-            }
-        }
+        oracle_keypoint_t r;
+        if (refine_one(p, (const float* const*)&P->d[o * nd], nd, P->w[o], P->h[o], kp, &r))
+            kv_push(out, r);
     }
+}
+
+int oracle_refine_single(const oracle_params_t* p, const float* dog, int w, int h, int nd,
+                         const oracle_keypoint_t* cand, oracle_keypoint_t* out) {
+    const float* dl[64];
+    if (nd < 3 || nd > 64) return 0;
+    for (int l = 0; l < nd; l++) dl[l] = dog + (size_t)l * w * h;
+    return refine_one(p, dl, nd, w, h, cand, out);
 }
 
 /* --------------------------------------------------------------------------
  * assign_orientations (src/sift.rs:605-726)
  * ------------------------------------------------------------------------ */
 
-static int orient_one(const oracle_params_t* p, const pyr_t* P, const oracle_keypoint_t* kp,
-                      oracle_keypoint_t* outv /* capacity 36 */) {
+/* The raw 36-bin histogram of one keypoint (src/sift.rs:614-671): 0 when the
+ * keypoint has no valid level or size. */
+static int orient_histogram(const oracle_params_t* p, const pyr_t* P, const oracle_keypoint_t* kp,
+                            float* hist /* 36 */) {
+    (void)p;
     int S = P->S, L = P->L;
     int o = kp->octave;
     int gl = kp->layer < 0 ? 0 : (kp->layer > S + 2 ? S + 2 : kp->layer); /* :618 */
+    memset(hist, 0, 36 * sizeof(float));
     if (o < 0 || o >= P->n_oct || gl >= L) return 0;
     const uint8_t* g = P->g[o * L + gl];
     int w = P->w[o], h = P->h[o];
@@ -647,8 +668,6 @@ static int orient_one(const oracle_params_t* p, const pyr_t* P, const oracle_key
     int R = f2i_sat(roundf(4.5f * so)); /* (3.0 * 1.5 * sigma_octave).round() */
     float ws = 1.5f * so;
     float denom = 2.0f * ws * ws;
-    float hist[36];
-    memset(hist, 0, sizeof hist);
     for (int dy = -R; dy <= R; dy++) {
         for (int dx = -R; dx <= R; dx++) {
             int xi = f2i_sat(roundf(xo + (float)dx));
@@ -669,12 +688,23 @@ static int orient_one(const oracle_params_t* p, const pyr_t* P, const oracle_key
             hist[bin] += mag * wt;
         }
     }
+    return 1;
+}
+
+/* Two passes of the circular [1, 1, 1] / 3 smoothing (src/sift.rs:677-685). */
+void oracle_smooth_orientation_histogram(float* hist /* 36 */) {
     for (int it = 0; it < 2; it++) { /* ORIENTATION_SMOOTHING_ITERATIONS */
         float prev[36];
         memcpy(prev, hist, sizeof prev);
         for (int i = 0; i < 36; i++)
             hist[i] = (prev[(i + 35) % 36] + prev[i] + prev[(i + 1) % 36]) / 3.0f;
     }
+}
+
+/* Peaks of a smoothed histogram (src/sift.rs:688-720): bins >= 0.8 max that are
+ * strictly above both circular neighbours, with a parabolic fit; angles in
+ * (-pi, pi], in ascending bin order. Returns their count. */
+int oracle_orientation_peaks(const float* hist /* 36 */, float* angles /* 36 */) {
     float mx = 0.0f;
     for (int i = 0; i < 36; i++) mx = fmaxf(mx, hist[i]);
     float thr = mx * 0.8f;
@@ -691,11 +721,22 @@ static int orient_one(const oracle_params_t* p, const pyr_t* P, const oracle_key
                 float a = center + off * step;
                 float fa = rem_euclid_f(a, 2.0f * PI_F);
                 if (fa > PI_F) fa = fa - 2.0f * PI_F;
-                outv[n] = *kp;
-                outv[n].angle = fa;
-                n++;
+                angles[n++] = fa;
             }
         }
+    }
+    return n;
+}
+
+static int orient_one(const oracle_params_t* p, const pyr_t* P, const oracle_keypoint_t* kp,
+                      oracle_keypoint_t* outv /* capacity 36 */) {
+    float hist[36], ang[36];
+    if (!orient_histogram(p, P, kp, hist)) return 0;
+    oracle_smooth_orientation_histogram(hist);
+    int n = oracle_orientation_peaks(hist, ang);
+    for (int i = 0; i < n; i++) {
+        outv[i] = *kp;
+        outv[i].angle = ang[i];
     }
     return n;
 }
@@ -722,7 +763,9 @@ void oracle_normalize_and_clip_descriptor(float* d, int n) {
     for (int i = 0; i < n; i++) d[i] *= inv2;
 }
 
-static void descriptor_one(const oracle_params_t* p, const pyr_t* P, const oracle_keypoint_t* kp,
+/* The 4 x 4 x 8 histogram of compute (src/sift.rs:927-1052), before
+ * normalize_and_clip_descriptor. */
+static void descriptor_raw(const oracle_params_t* p, const pyr_t* P, const oracle_keypoint_t* kp,
                            float* hist /* 128, zeroed */) {
     int S = P->S, L = P->L;
     int o = kp->octave;
@@ -776,7 +819,12 @@ static void descriptor_one(const oracle_params_t* p, const pyr_t* P, const oracl
                     }
         }
     }
-    oracle_normalize_and_clip_descriptor(hist, 128);
+}
+
+static void descriptor_one(const oracle_params_t* p, const pyr_t* P, const oracle_keypoint_t* kp,
+                           float* hist /* 128, zeroed */) {
+    descriptor_raw(p, P, kp, hist);
+    oracle_normalize_and_clip_descriptor(hist, 128); /* :1054 */
 }
 
 /* --------------------------------------------------------------------------
@@ -880,6 +928,29 @@ int oracle_orientation_single(const oracle_params_t* p, const uint8_t* g, int w,
     int n = orient_one(p, &P, &k, out);
     for (int i = 0; i < n; i++) angles[i] = out[i].angle;
     return n;
+}
+
+int oracle_orientation_histogram_single(const oracle_params_t* p, const uint8_t* g, int w, int h,
+                                        const oracle_keypoint_t* kp, float* hist /* 36 */) {
+    pyr_t P;
+    uint8_t* gp[64];
+    int ws[1], hs[1];
+    single_level_pyr(p, g, w, h, &P, gp, ws, hs);
+    oracle_keypoint_t k = *kp;
+    k.octave = 0;
+    return orient_histogram(p, &P, &k, hist);
+}
+
+void oracle_descriptor_raw_single(const oracle_params_t* p, const uint8_t* g, int w, int h,
+                                  const oracle_keypoint_t* kp, float* hist /* 128 */) {
+    pyr_t P;
+    uint8_t* gp[64];
+    int ws[1], hs[1];
+    single_level_pyr(p, g, w, h, &P, gp, ws, hs);
+    oracle_keypoint_t k = *kp;
+    k.octave = 0;
+    memset(hist, 0, 128 * sizeof(float));
+    descriptor_raw(p, &P, &k, hist);
 }
 
 void oracle_descriptor_single(const oracle_params_t* p, const uint8_t* g, int w, int h,

@@ -122,6 +122,22 @@ int oracle_orientation_single(const oracle_params_t* p, const uint8_t* g, int w,
 void oracle_descriptor_single(const oracle_params_t* p, const uint8_t* g, int w, int h,
                               const oracle_keypoint_t* kp, float* desc);
 
+/* The pieces of those two, for closed-form checks: the raw orientation
+ * histogram (before smoothing), the two smoothing passes, the peak rule, and
+ * the descriptor histogram before normalize_and_clip_descriptor. */
+int oracle_orientation_histogram_single(const oracle_params_t* p, const uint8_t* g, int w, int h,
+                                        const oracle_keypoint_t* kp, float* hist);
+void oracle_smooth_orientation_histogram(float* hist);
+int oracle_orientation_peaks(const float* hist, float* angles);
+void oracle_descriptor_raw_single(const oracle_params_t* p, const uint8_t* g, int w, int h,
+                                  const oracle_keypoint_t* kp, float* hist);
+
+/* refine_and_filter_extrema (src/sift.rs:299-474) of one candidate on one
+ * octave's DoG layers `dog` (nd layers of w x h f32, layer-major). Returns 1 and
+ * writes *out when the candidate passes, 0 when it is dropped. */
+int oracle_refine_single(const oracle_params_t* p, const float* dog, int w, int h, int nd,
+                         const oracle_keypoint_t* cand, oracle_keypoint_t* out);
+
 #ifdef __cplusplus
 }
 #endif
