@@ -23,6 +23,7 @@ sys.path.insert(0, os.path.join(REPO, "sift-wgpu_amd"))
 
 METRIC = "1080p frames/s & MPix/s (batched); p50/p99 latency on mixed-size stream"
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # non-packed f32 lane operations per second (39.3 T)
 
 
 def parse():
@@ -37,11 +38,16 @@ def parse():
     ap.add_argument("--no-profile", action="store_true", help="skip the HIP-event roofline")
     ap.add_argument("--mixed-batch", type=int, default=64,
                     help="frames in the mixed-size batch leg (0 disables it)")
-    ap.add_argument("--serve-seconds", type=float, default=10.0,
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the served open-loop stream (0 disables it)")
+    ap.add_argument("--capacity-seconds", type=float, default=5.0,
+                    help="length of the closed-loop served-capacity leg (0 disables it)")
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="requests/s of the stream")
     ap.add_argument("--traffic", default=os.path.join(REPO, "profiles", "traffic_1080p_b64.txt"),
                     help="rocprofv3 FETCH/WRITE summary giving HBM bytes per launch")
+    ap.add_argument("--configs", action="store_true",
+                    help="only measure every workload configuration (CPU oracle on all threads and "
+                         "one, HIP per-image and batched, blur share of HBM peak) and exit")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (for scaling "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -96,38 +102,50 @@ def mixed_batch(n, step):
             "keypoints": int(sum(counts))}
 
 
+SERVED_POOL = [32, 24, 16, 8]  # distinct frames per size of the served legs (80 in all)
+
+
+def served_setup(n, rate, seed=7):
+    """Requests 0..n-1 of the served workload: Poisson arrivals, the 40/30/20/10%
+    size mix, request j's frame seeded 100000 + j for the first SERVED_POOL[k] of
+    each size and reused in rotation after that; octaves floor(log2 min) - 3."""
+    import imgen
+    import sift_hip
+    arrivals, which = imgen.served_stream(n, rate, seed)
+    src, frames = imgen.served_frames(which, SERVED_POOL)
+    per_size = [sift_hip.Sift(1.6, imgen.octaves_for(w, h), 3, 0.5, 0.04, 10.0)
+                for (w, h) in imgen.SERVED_SIZES]
+    return arrivals, which, src, frames, per_size
+
+
+def _warm(srv, frames, which_of, per_size):
+    """Bursts of every pooled frame, so that every lane has built its per-size
+    tables and grown its staging before the clock starts."""
+    tag = 10**9
+    for _ in range(3):
+        for j, img in frames.items():
+            srv.submit(img, tag, per_size[which_of[j]])
+            tag += 1
+    got = 0
+    while got < tag - 10**9:
+        got += len(srv.poll(timeout_ms=1000))
+
+
 def served(seconds, rate, seed=7):
-    """Open-loop Poisson stream: sizes 40/30/20/10% of 640x480 / 1280x720 /
-    1920x1080 / 3840x2160 (octaves floor(log2(min))-3), host frames in, host
-    results out; latency = completion - submit."""
+    """Open-loop Poisson stream (host frames in, host results out); latency =
+    completion - submit, by size and overall."""
     import threading
     import numpy as np
     import imgen
     import sift_hip
-    sizes = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
-    mix = [0.4, 0.3, 0.2, 0.1]
-    pool = {sz: [imgen.generic_image(sz[0], sz[1], seed=100000 + 10 * k + i) for i in range(4)]
-            for k, sz in enumerate(sizes)}
-    rng = np.random.default_rng(seed)
+    sizes = imgen.SERVED_SIZES
     n = int(seconds * rate)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
-    which = rng.choice(4, size=n, p=mix)
-    per_size = {sz: sift_hip.Sift(1.6, imgen.octaves_for(*sz), 3, 0.5, 0.04, 10.0) for sz in sizes}
-    sift = sift_hip.Sift.default()
+    arrivals, which, src, frames, per_size = served_setup(n, rate, seed)
+    which_of = {int(src[j]): int(which[j]) for j in range(n)}
     lat = {sz: [] for sz in sizes}
     done = []
-    with sift_hip.SiftServer(sift, hbm_budget_bytes=32 << 30, max_batch=64) as srv:
-        # warm-up: bursts of every pooled frame, so both lanes have built their
-        # per-size tables and grown their staging before the clock starts
-        tag = 10**9
-        for _ in range(3):
-            for sz in sizes:
-                for img in pool[sz]:
-                    srv.submit(img, tag, per_size[sz])
-                    tag += 1
-        got = 0
-        while got < tag - 10**9:
-            got += len(srv.poll(timeout_ms=1000))
+    with sift_hip.SiftServer(sift_hip.Sift.default(), hbm_budget_bytes=32 << 30, max_batch=64) as srv:
+        _warm(srv, frames, which_of, per_size)
         t_start = time.perf_counter()
 
         def producer():
@@ -135,8 +153,7 @@ def served(seconds, rate, seed=7):
                 delay = t_start + arrivals[j] - time.perf_counter()
                 if delay > 0:
                     time.sleep(delay)
-                sz = sizes[which[j]]
-                srv.submit(pool[sz][j % 4], j, per_size[sz])
+                srv.submit(frames[int(src[j])], j, per_size[int(which[j])])
         th = threading.Thread(target=producer)
         th.start()
         while len(done) < n:
@@ -154,12 +171,178 @@ def served(seconds, rate, seed=7):
     mp = sum(sizes[which[t]][0] * sizes[which[t]][1] for t, _, _ in done) / 1e6
     return {"requests": n, "completed": len(done), "failed": sum(1 for d in done if d[1] != 0),
             "rate_req_per_s": rate, "seconds": seconds, "frames_per_s": round(len(done) / wall, 1),
-            "mpix_per_s": round(mp / wall, 1),
+            "mpix_per_s": round(mp / wall, 1), "distinct_frames": len(frames),
             "p50_ms": q(all_lat, 50), "p99_ms": q(all_lat, 99), "p99_9_ms": q(all_lat, 99.9),
             "by_size": {f"{w}x{h}": {"n": len(lat[(w, h)]), "p50_ms": q(lat[(w, h)], 50),
-                                     "p99_ms": q(lat[(w, h)], 99)} for (w, h) in sizes},
+                                     "p99_ms": q(lat[(w, h)], 99), "p99_9_ms": q(lat[(w, h)], 99.9)}
+                        for (w, h) in sizes},
             "octaves": {f"{w}x{h}": imgen.octaves_for(w, h) for (w, h) in sizes},
             "io": "host frames in, host results out (PCIe included)"}
+
+
+def served_capacity(seconds, window=256, seed=8):
+    """Closed loop: the served mix submitted as fast as completions allow (up to
+    `window` requests outstanding) for `seconds`, against the same frames run one
+    at a time through the plain per-image path (sift_hip_detect, host in / host
+    out). Both are frames/s of the mix."""
+    import numpy as np
+    import imgen
+    import sift_hip
+    n_pool = 4096
+    _, which, src, frames, per_size = served_setup(n_pool, 1000.0, seed)
+    which_of = {int(src[j]): int(which[j]) for j in range(n_pool)}
+    sizes = imgen.SERVED_SIZES
+    with sift_hip.SiftServer(sift_hip.Sift.default(), hbm_budget_bytes=32 << 30, max_batch=64) as srv:
+        _warm(srv, frames, which_of, per_size)
+        t0 = time.perf_counter()
+        sent = got = 0
+        px = 0
+        while time.perf_counter() - t0 < seconds:
+            while sent - got < window:
+                j = sent % n_pool
+                srv.submit(frames[int(src[j])], sent, per_size[int(which[j])])
+                sent += 1
+            for tag, st, *_ in srv.poll(timeout_ms=100):
+                got += 1
+                w, h = sizes[which[tag % n_pool]]
+                px += w * h
+        while got < sent:
+            for tag, st, *_ in srv.poll(timeout_ms=1000):
+                got += 1
+                w, h = sizes[which[tag % n_pool]]
+                px += w * h
+        wall = time.perf_counter() - t0
+    # the plain per-image path on the same request sequence (bounded sample)
+    for j in range(8):
+        per_size[int(which[j])].detect_and_compute(frames[int(src[j])])
+    t1 = time.perf_counter()
+    k = 0
+    while time.perf_counter() - t1 < min(seconds, 3.0):
+        j = k % n_pool
+        per_size[int(which[j])].detect_and_compute(frames[int(src[j])])
+        k += 1
+    tp = (time.perf_counter() - t1) / k
+    fps = got / wall
+    return {"frames_per_s": round(fps, 1), "mpix_per_s": round(px / 1e6 / wall, 1),
+            "outstanding": window, "seconds": round(wall, 2), "frames": got,
+            "per_image_frames_per_s": round(1.0 / tp, 1), "served_vs_per_image": round(fps * tp, 3),
+            "io": "host frames in, host results out (PCIe included), both legs"}
+
+
+def tuning_leg(sift, batch, steps):
+    """The batched step with the tuning table (blur_tune.json) and without it (the
+    built-in blur forms), each on a context of its own, alternating, frames/s."""
+    import torch
+    import sift_hip
+    os.environ["SIFT_HIP_TUNE_TABLE"] = "0"
+    plain = sift_hip.new_context()
+    del os.environ["SIFT_HIP_TUNE_TABLE"]
+    tuned = sift_hip.new_context()
+    table = os.path.join(REPO, "sift-wgpu_amd", "blur_tune.json")
+    res = {"table": os.path.relpath(table, REPO) if os.path.exists(table) else None}
+    times = {"table": [], "built_in": []}
+    for ctx in (tuned, plain):
+        for _ in range(3):
+            sift.detect_batch_device(batch, ctx=ctx)
+    for _ in range(3):
+        for name, ctx in (("table", tuned), ("built_in", plain)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(max(steps // 2, 5)):
+                sift.detect_batch_device(batch, ctx=ctx)
+            torch.cuda.synchronize()
+            times[name].append((time.perf_counter() - t0) / max(steps // 2, 5))
+    for name, t in times.items():
+        res[name + "_frames_per_s"] = round(len(batch) / statistics.median(t), 1)
+    sift_hip.free_context(tuned)
+    sift_hip.free_context(plain)
+    return res
+
+
+def config_rows():
+    """One row per configuration: 512x512, 1920x1080, 3840x2160 (single frames of
+    the generator's workloads, batched as copies of distinct seeds), and the
+    64-frame uneven batch. CPU: the oracle, median of a few runs on all threads
+    and on one. HIP: per-image = one device-resident frame per call; batched =
+    one call over the batch; blur_hbm_frac = algorithmic bytes of the blur
+    kernels (u8 levels in and out, b_G = 1) over their event time, of 8 TB/s."""
+    import torch
+    import imgen
+    import sift_hip
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    ncpu = os.cpu_count() or 1
+    threads = min(ncpu, int(os.environ.get("OMP_NUM_THREADS", ncpu)))
+    rows = {}
+
+    def cpu_ms(img, octv, ct, nthreads, runs):
+        p = oracle.params(1.6, octv, 3, 0.5, ct, 10.0)
+        oracle.detect_and_compute(p, img, nthreads=nthreads)
+        t = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            oracle.detect_and_compute(p, img, nthreads=nthreads)
+            t.append(time.perf_counter() - t0)
+        return statistics.median(t) * 1e3
+
+    def hip_legs(imgs, params):
+        dev = [torch.from_numpy(a).to("cuda") for a in imgs]
+        torch.cuda.synchronize()
+        items = [(t.data_ptr(), a.shape[1], a.shape[0], a.shape[1]) for t, a in zip(dev, imgs)]
+        s = sift_hip.Sift.default()
+        for _ in range(2):
+            s.detect_batch_device(items, params)
+        sift_hip.profile(True)
+        s.detect_batch_device(items, params)
+        prof = sift_hip.profile_read()
+        sift_hip.profile(False)
+        blur = [v for k, v in prof.items() if k.startswith("k_blur") or k == "k_copy_level"]
+        by = sum(v[2] for v in blur)
+        ms = sum(v[1] for v in blur)
+        reps = 10
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            s.detect_batch_device(items, params)
+        torch.cuda.synchronize()
+        tb = (time.perf_counter() - t0) / reps
+        lat = []
+        for i in range(20):
+            k = i % len(items)
+            t1 = time.perf_counter()
+            params[k].detect_batch_device([items[k]])
+            lat.append(time.perf_counter() - t1)
+        tp = statistics.median(lat)
+        mpix = sum(a.size for a in imgs) / 1e6
+        return {"per_image_ms": round(tp * 1e3, 3), "per_image_fps": round(1 / tp, 1),
+                "batched_fps": round(len(imgs) / tb, 1), "batched_mpix_per_s": round(mpix / tb, 1),
+                "batch_frames": len(imgs), "blur_hbm_frac": round(by / (ms * 1e-3) / 8e12, 4) if ms else None}
+
+    for name, nb, runs in (("512x512", 64, 7), ("1920x1080", 64, 5), ("3840x2160", 16, 3)):
+        w, h, b, r, noise, seed, octv, ct = imgen.CONFIGS[name]
+        img = imgen.synth_image(w, h, b, r, noise, seed)
+        c_all = cpu_ms(img, octv, ct, threads, runs)
+        c_one = cpu_ms(img, octv, ct, 1, max(2, runs // 2))
+        batch = [imgen.synth_image(w, h, b, r, noise, seed + 1000 * i) for i in range(nb)]
+        params = [sift_hip.Sift(1.6, octv, 3, 0.5, ct, 10.0)] * nb
+        row = {"cpu_ms": round(c_all, 1), "cpu_fps": round(1e3 / c_all, 3),
+               "cpu_mpix_per_s": round(w * h / 1e3 / c_all, 3), "cpu_threads": threads,
+               "cpu_1thread_ms": round(c_one, 1)}
+        row.update(hip_legs(batch, params))
+        rows[name] = row
+        print(json.dumps({name: row}), flush=True)
+    shapes = imgen.batch_shapes(64)
+    imgs = [imgen.generic_image(w, h, seed=1000 + i) for i, (w, h) in enumerate(shapes)]
+    params = [sift_hip.Sift(1.6, imgen.octaves_for(w, h), 3, 0.5, 0.04, 10.0) for (w, h) in shapes]
+    cpu = 0.0
+    for i in range(len(imgs)):  # the reference's way: the frames one after another
+        cpu += cpu_ms(imgs[i], imgen.octaves_for(*shapes[i]), 0.04, threads, 1)
+    row = {"cpu_ms": round(cpu, 1), "cpu_fps": round(len(imgs) * 1e3 / cpu, 3),
+           "cpu_mpix_per_s": round(sum(a.size for a in imgs) / 1e3 / cpu, 3), "cpu_threads": threads}
+    row.update(hip_legs(imgs, params))
+    rows["batch64"] = row
+    print(json.dumps({"batch64": row}), flush=True)
+    return {"cpu_model": cpu_model(), "rows": rows}
 
 
 def frames(n):
@@ -168,23 +351,49 @@ def frames(n):
     return [imgen.synth_image(w, h, b, r, noise, seed + 1000 * i) for i in range(n)], octv, ct
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(img, octv, ct, budget_s):
+    """The oracle (oracle/sift_oracle.c: the reference CPU path restated in C,
+    OpenMP where the reference uses rayon) on one 1080p frame: the median of
+    repeated runs with the box's thread share (OMP_NUM_THREADS, else all cores),
+    and with one thread."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
     p = oracle.params(1.6, octv, 3, 0.5, ct, 10.0)
-    threads = min(16, os.cpu_count() or 1)
-    oracle.detect_and_compute(p, img, nthreads=threads)  # warm-up
-    times = []
-    t_end = time.perf_counter() + budget_s
-    while time.perf_counter() < t_end or len(times) < 3:
-        t0 = time.perf_counter()
-        oracle.detect_and_compute(p, img, nthreads=threads)
-        times.append(time.perf_counter() - t0)
-        if len(times) >= 50:
-            break
-    med = statistics.median(times)
+    ncpu = os.cpu_count() or 1
+    threads = min(ncpu, int(os.environ.get("OMP_NUM_THREADS", ncpu)))
+
+    def sample(nthreads, budget):
+        oracle.detect_and_compute(p, img, nthreads=nthreads)  # warm-up
+        times = []
+        t_end = time.perf_counter() + budget
+        while time.perf_counter() < t_end or len(times) < 3:
+            t0 = time.perf_counter()
+            oracle.detect_and_compute(p, img, nthreads=nthreads)
+            times.append(time.perf_counter() - t0)
+            if len(times) >= 50:
+                break
+        return statistics.median(times), len(times)
+
+    med, n = sample(threads, budget_s * 0.6)
+    med1, n1 = sample(1, budget_s * 0.4)
+    model = cpu_model()
     return {"value": round(1.0 / med, 4), "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"{len(times)} runs of one 1920x1080 frame (median {med * 1e3:.1f} ms), "
+            "cpu_model": model, "cpu_count": ncpu,
+            "one_thread": {"value": round(1.0 / med1, 4), "unit": "frames/s", "cores": 1,
+                           "ms_per_frame": round(med1 * 1e3, 1), "runs": n1},
+            "mpix_per_s": round(1920 * 1080 / 1e6 / med, 3),
+            "sample": f"{n} runs of one 1920x1080 frame on {threads} threads (median {med * 1e3:.1f} ms) "
+                      f"and {n1} on 1 thread (median {med1 * 1e3:.1f} ms), {model}; "
                       f"oracle/sift_oracle.c with OpenMP where the reference uses rayon"}
 
 
@@ -196,6 +405,11 @@ def main():
         sys.exit(2)
     if args.copy_calib > 0:
         copy_calib(args.copy_calib)
+        return
+    if args.configs:
+        import torch  # noqa: F401  (touches the GPU before the library loads)
+        torch.cuda.init()
+        print(json.dumps(config_rows()))
         return
     import torch
     import sift_hip
@@ -286,13 +500,22 @@ def main():
         # (blur / resize / extremum scan; the keypoint stages are gather-bound),
         # timed with HIP events in the timed steps
         dom = roof_kernel
-        n, ms, by = stats[dom]
+        n, ms, by, _ = stats[dom]
         avg_s = ms * 1e-3 / n
         ach = by / n / avg_s / 1e9
         out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1),
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                            "traffic": None, "bytes_per_launch": round(by / n),
                            "avg_launch_us": round(avg_s * 1e6, 3)}
+        ops = stats[dom][3]
+        if ops > 0:
+            # the exact arithmetic (a multiply and an add per tap, no FMA, no packed
+            # FP32) against the non-packed f32 VALU rate: 256 CUs x 64 lanes x 2.4 GHz
+            vach = ops / n / avg_s / 1e12
+            out["roofline"]["valu_achieved"] = round(vach, 3)
+            out["roofline"]["valu_peak"] = VALU_PEAK_TOPS
+            out["roofline"]["valu_unit"] = "T f32 ops/s"
+            out["roofline"]["valu_frac"] = round(vach / VALU_PEAK_TOPS, 4)
         if os.path.exists(args.traffic):
             tr = {}
             for line in open(args.traffic):
@@ -302,10 +525,13 @@ def main():
             if dom in tr:
                 out["roofline"]["traffic"] = round(tr[dom][0] + tr[dom][1])
                 out["roofline"]["traffic_read_write"] = [round(tr[dom][0]), round(tr[dom][1])]
+    out["tuning"] = tuning_leg(sift, batch, args.steps)
     if args.mixed_batch > 0:
         out["mixed_batch"] = mixed_batch(args.mixed_batch, step)
     if args.serve_seconds > 0:
         out["served"] = served(args.serve_seconds, args.serve_rate)
+    if args.capacity_seconds > 0:
+        out["served_capacity"] = served_capacity(args.capacity_seconds)
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(imgs[0], octv, ct, args.cpu_seconds)
     print(json.dumps(out))

@@ -145,18 +145,40 @@ void sift_stages_free(sift_stages_t* s);
 
 /* ---- Kernel timing --------------------------------------------------------
  * When enabled, every kernel launch of the context is bracketed by HIP events
- * on the context's stream; durations and the launch's algorithmic HBM bytes
- * (u8 levels in and out for the blur and resize passes, levels read plus bitmask
- * written for the extremum scan; 0 for the gather-bound keypoint stages) are
- * accumulated per kernel name. Enabling resets the counters. */
+ * on the context's stream; durations, the launch's algorithmic HBM bytes (u8
+ * levels in and out for the blur and resize passes, levels read plus bitmask
+ * written for the extremum scan; 0 for the gather-bound keypoint stages) and
+ * its prescribed f32 operations (a multiply and an add per Gaussian tap or
+ * Lanczos weight and output pixel, the reference's arithmetic, which the
+ * kernels keep without FMA contraction; 0 for other kernels) are accumulated
+ * per kernel name. Enabling resets the counters. */
 typedef struct sift_kernel_stat_t {
     char name[32];
     uint64_t launches;
     double total_ms;
     double bytes;
+    double ops;
 } sift_kernel_stat_t;
 
 int sift_hip_profile(sift_hip_ctx* ctx, int enable);
+
+/* ---- Blur tuning ----------------------------------------------------------
+ * No counterpart in the reference: the measurement behind the tuning table that
+ * chooses each pyramid level's blur form (its tile shape and outputs per lane)
+ * by level width, height and tap count. Times one separable blur of sigma on
+ * `jobs` levels of width x height in one launch per pass, with the given
+ * horizontal form ("h8", "h2", "hsmall", "hgeneric") and vertical form ("v",
+ * "vnarrow", "vsmall", "vgeneric"; NULL or "" = the built-in choice), and
+ * writes the median over `reps` of the horizontal and the vertical pass, in ms,
+ * to out_ms[0] and out_ms[1], and the forms used (indices in the two lists
+ * above) to out_forms[0] and [1] (may be NULL). SIFT_ERR_INVALID for a form not
+ * valid for the level. The table the library loads at context creation is blur_tune.json
+ * next to it (or the file SIFT_HIP_TUNE_TABLE names; "0" = none); its entries
+ * are [width, height, taps, many, "<h form>", "<v form>"], many = 1 for launches
+ * carrying a batch of 8 or more frames. */
+int sift_hip_tune_blur(sift_hip_ctx* ctx, uint32_t width, uint32_t height, float sigma,
+                       uint32_t jobs, const char* h_form, const char* v_form, uint32_t reps,
+                       double* out_ms, int* out_forms);
 /* Restricts the timing to launches of one kernel (its name as in
  * sift_kernel_stat_t, e.g. "k_blur_h<8>"), so that the events add little to
  * the timed work; NULL or "" times every kernel. Unknown name: SIFT_ERR_INVALID. */

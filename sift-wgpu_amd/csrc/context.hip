@@ -5,7 +5,9 @@
 // with -ffp-contract=off, SSE f32, glibc libm): level sigmas
 // (src/sift.rs:132-147), initial blur (src/sift.rs:1205-1209), imageproc's
 // Gaussian taps and image's Lanczos3 weights. These feed the kernels as tables.
+#include <dlfcn.h>
 #include <math.h>
+#include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
@@ -85,14 +87,16 @@ float lanczos3_f(float x) { return fabsf(x) < 3.0f ? sinc_f(x) * sinc_f(x / 3.0f
 
 namespace sh {
 
+int set_error(int code, const char* msg) { return fail(code, msg); }
+
 // ---------------------------------------------------------------------------
 // Table store: taps and Lanczos tables in one growing device buffer.
 // ---------------------------------------------------------------------------
 int TableStore::ensure_uploaded(hipStream_t stream) {
     if (host.size() == uploaded) return SIFT_OK;
     if (host.size() > dev_cap) {
-        // rare: the first 16 MiB hold the tables of many parameter sets and sizes
-        size_t ncap = std::max(host.size() * 2, (size_t)16 << 20);
+        // rare: the first 2 MiB hold the tables of many parameter sets and sizes
+        size_t ncap = std::max(host.size() * 2, (size_t)2 << 20);
         void* nd = nullptr;
         HIP_TRY(hipMalloc(&nd, ncap));
         if (dev) {
@@ -255,7 +259,12 @@ void Context::clear_graphs() {
 // ---------------------------------------------------------------------------
 int Context::ensure_arena(size_t bytes) {
     if (bytes <= arena_cap) return SIFT_OK;
-    size_t ncap = std::max(bytes + bytes / 4, (size_t)64 << 20);
+    if (arena_limit && bytes > arena_limit)
+        return fail(SIFT_ERR_NOMEM, "batch needs " + std::to_string(bytes) +
+                                        " device bytes, more than this lane's share of the HBM budget (" +
+                                        std::to_string(arena_limit) + ")");
+    size_t ncap = std::max(bytes + bytes / 4, arena_floor);
+    if (arena_limit) ncap = std::min(ncap, arena_limit);
     if (arena) {
         HIP_TRY(hipStreamSynchronize(stream));
         clear_graphs();  // they address the old arena
@@ -349,6 +358,7 @@ struct LaunchRec {
     int njobs, tiles, max_pad;
     int variant;     // k_blur_h: outputs per lane; k_blur_v_narrow: staged rows
     double bytes;    // algorithmic HBM bytes of the launch
+    double ops;      // f32 operations the reference's arithmetic prescribes (profile ops)
 };
 
 enum KernelId {
@@ -372,6 +382,19 @@ double blur_job_bytes(const BlurJob& j, int kind) {
     if (kind == K_COPY && j.dog) return a * (2.0 + 4.0);
     return a * 2.0;
 }
+// A multiply and an add per Gaussian tap (Lanczos weight) and output pixel:
+// the reference's f32 arithmetic, which the kernels keep without contraction.
+double blur_job_ops(const BlurJob& j, int kind) {
+    if (kind == K_COPY) return 0.0;
+    return 2.0 * (double)j.w * j.h * j.ntaps;
+}
+double resize_job_ops(const ResizeJob& j, int kind) {
+    const double v = 2.0 * (double)j.in_w * j.out_h * (kind == K_RESIZE_FUSED ? j.vmaxtaps : j.maxtaps);
+    const double h = 2.0 * (double)j.out_w * j.out_h * j.maxtaps;
+    if (kind == K_RESIZE_FUSED) return v + h;
+    return kind == K_RESIZE_V ? v : h;
+}
+
 double luma_job_bytes(const LumaJob& j) {
     return (double)j.w * j.h * (pixel_bytes((uint32_t)j.format) + 1);
 }
@@ -381,7 +404,121 @@ double resize_job_bytes(const ResizeJob& j, int kind) {
     return (double)j.in_w * j.out_h * 4.0 + (double)j.out_w * j.out_h;
 }
 
+// ---------------------------------------------------------------------------
+// Blur pass forms: the choices of the tuning table
+// ---------------------------------------------------------------------------
+// Horizontal: row tiles of 512 or 128 columns x 4 rows, 8 or 2 outputs per
+// lane (k_blur_h<8> / <2>); 64 x 4 tiles with one output per lane
+// (k_blur_h_small); one row of 256 outputs per work-group (k_blur_h_generic).
+// Vertical: 256 x 16 tiles, 4 x 4 outputs per lane (k_blur_v); 64 x 16 tiles
+// with the input rows staged in LDS (k_blur_v_narrow); 64 x 4 tiles, one output
+// per lane (k_blur_v_small); one row per work-group (k_blur_v_generic). Every
+// form computes the same exact sums, so any valid choice gives the same level.
+const char* const kBlurHNames[BH_COUNT] = {"h8", "h2", "hsmall", "hgeneric"};
+const char* const kBlurVNames[BV_COUNT] = {"v", "vnarrow", "vsmall", "vgeneric"};
+
+int blur_form_id(const char* name, bool vertical) {
+    if (!name) return -1;
+    const char* const* names = vertical ? kBlurVNames : kBlurHNames;
+    const int n = vertical ? BV_COUNT : BH_COUNT;
+    for (int i = 0; i < n; i++)
+        if (strcmp(names[i], name) == 0) return i;
+    return -1;
+}
+
+// LDS of the 4 staged rows of k_blur_h<px> (padded as it pads them)
+int blur_h_row_lds(int px, int npad) {
+    const int hrow = 64 * px + npad + 8;
+    return (hrow + 8 + ((hrow + 8) >> (px == 8 ? 3 : 5)) + 1) * 4 * (int)sizeof(float);
+}
+
+bool blur_h_valid(int form, int w, int h, int ntaps, int npad) {
+    (void)w;
+    (void)h;
+    const int n4 = (ntaps + 3) & ~3;
+    switch (form) {
+        case BH_W8: return blur_h_row_lds(8, npad) <= 65536;
+        case BH_W2: return blur_h_row_lds(2, npad) <= 65536;
+        case BH_SMALL: return n4 <= kSmallMaxTaps4;
+        case BH_GENERIC: return true;
+        default: return false;
+    }
+}
+
+bool blur_v_valid(int form, int w, int h, int ntaps) {
+    (void)w;
+    const int n4 = (ntaps + 3) & ~3, r = ntaps / 2;
+    switch (form) {
+        case BV_WIDE: return true;
+        case BV_NARROW: return std::min(h, 16 + 2 * r) * 64 <= 65536;
+        case BV_SMALL: return h <= kSmallMaxRows && n4 <= kSmallMaxTaps4;
+        case BV_GENERIC: return true;
+        default: return false;
+    }
+}
+
+// The built-in choice: small levels and long kernels on narrow levels take one
+// output per lane; wide levels the 8-output row tiles and the 4 x 4 column tiles.
+std::pair<int, int> blur_heuristic(int w, int h, int ntaps, int npad) {
+    const int px = w >= 384 ? 8 : 2;
+    const int n4 = (ntaps + 3) & ~3, r = ntaps / 2;
+    const bool small_level = (long long)w * h <= 65536 || r >= 24;
+    int hf, vf;
+    if (w < 384 && n4 <= kSmallMaxTaps4 && small_level) hf = BH_SMALL;
+    else if (blur_h_row_lds(px, npad) <= 65536) hf = px == 8 ? BH_W8 : BH_W2;
+    else hf = BH_GENERIC;
+    if (h <= kSmallMaxRows && n4 <= kSmallMaxTaps4 && (small_level || w < 192 || ntaps > 64)) vf = BV_SMALL;
+    else if (w >= 192 && ntaps <= 64) vf = BV_WIDE;
+    else if (blur_v_valid(BV_NARROW, w, h, ntaps)) vf = BV_NARROW;
+    else vf = BV_GENERIC;
+    return {hf, vf};
+}
+
+// Tile counts of a job in a form.
+void blur_h_tiles(BlurJob& J, int form) {
+    const int w = J.w, h = J.h;
+    switch (form) {
+        case BH_W8: J.tiles_x = (w + 511) / 512; J.tiles = J.tiles_x * ((h + 3) / 4); break;
+        case BH_W2: J.tiles_x = (w + 127) / 128; J.tiles = J.tiles_x * ((h + 3) / 4); break;
+        case BH_SMALL: J.tiles_x = (w + 63) / 64; J.tiles = J.tiles_x * ((h + 3) / 4); break;
+        default: J.tiles_x = (w + 255) / 256; J.tiles = J.tiles_x * h; break;
+    }
+}
+void blur_v_tiles(BlurJob& J, int form) {
+    const int w = J.w, h = J.h;
+    switch (form) {
+        case BV_WIDE: J.tiles_x = (w + 255) / 256; J.tiles = J.tiles_x * ((h + 15) / 16); break;
+        case BV_NARROW: J.tiles_x = (w + 63) / 64; J.tiles = J.tiles_x * ((h + 15) / 16); break;
+        case BV_SMALL: J.tiles_x = (w + 63) / 64; J.tiles = J.tiles_x * ((h + 3) / 4); break;
+        default: J.tiles_x = (w + 255) / 256; J.tiles = J.tiles_x * h; break;
+    }
+}
+
+// SIFT_HIP_BLUR_FORCE="<h form>,<v form>" (read on every call): forms used for
+// every job they are valid for, so that tests can compare them.
+std::pair<int, int> blur_forced() {
+    const char* e = getenv("SIFT_HIP_BLUR_FORCE");
+    if (!e || !e[0]) return {-1, -1};
+    std::string s(e);
+    const size_t c = s.find(',');
+    const std::string a = s.substr(0, c), b = c == std::string::npos ? "" : s.substr(c + 1);
+    return {blur_form_id(a.c_str(), false), blur_form_id(b.c_str(), true)};
+}
+
 }  // namespace
+
+std::pair<int, int> Context::blur_choice(int w, int h, int ntaps, int npad, bool many,
+                                         std::pair<int, int> forced) const {
+    std::pair<int, int> c = blur_heuristic(w, h, ntaps, npad);
+    auto it = tune.find(TuneKey{w, h, ntaps, many ? 1 : 0});
+    if (it != tune.end()) {
+        if (blur_h_valid(it->second.first, w, h, ntaps, npad)) c.first = it->second.first;
+        if (blur_v_valid(it->second.second, w, h, ntaps)) c.second = it->second.second;
+    }
+    if (forced.first >= 0 && blur_h_valid(forced.first, w, h, ntaps, npad)) c.first = forced.first;
+    if (forced.second >= 0 && blur_v_valid(forced.second, w, h, ntaps)) c.second = forced.second;
+    return c;
+}
 
 hipEvent_t Context::prof_event() {
     if (!prof_events.empty()) {
@@ -403,12 +540,12 @@ void Context::prof_begin(int kernel, double bytes, hipEvent_t* a) {
     if (*a) (void)hipEventRecord(*a, stream);
 }
 
-void Context::prof_end(int kernel, double bytes, hipEvent_t a) {
+void Context::prof_end(int kernel, double bytes, hipEvent_t a, double ops) {
     if (!profiling || !a) return;
     hipEvent_t b = prof_event();
     if (!b) return;
     (void)hipEventRecord(b, stream);
-    prof_pending.push_back({kernel, bytes, a, b});
+    prof_pending.push_back({kernel, bytes, ops, a, b});
 }
 
 void Context::prof_resolve() {
@@ -419,6 +556,7 @@ void Context::prof_resolve() {
             S.launches++;
             S.total_ms += ms;
             S.bytes += p.bytes;
+            S.ops += p.ops;
         }
         prof_events.push_back(p.a);
         prof_events.push_back(p.b);
@@ -551,7 +689,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     const int cap = (int)cand_cap, kcap = (int)kp_cap;
 
     const size_t bits_off = C.take((size_t)std::max<long long>(nbits, 1) * 8);
-    const size_t counts_off = C.take((size_t)(nslots + 2) * 4);  // + the descriptor's list counts
+    const size_t counts_off = C.take((size_t)(nslots + kCountExtra) * 4);  // + lists and queue heads
     const size_t offs_off = C.take((size_t)(nslots + 2) * 4);
     const size_t scantmp1_off = C.take((size_t)(nslots / kScanTileElems + 4) * 4);
     const size_t cand_off = C.take((size_t)cap * sizeof(Cand));
@@ -588,6 +726,14 @@ int Context::run(const Batch& B, RunOutputs* outs) {
                               octs.size() * sizeof(OctaveDesc) + (size_t)n * sizeof(ImageParams) +
                               (size_t)(n + 1) * sizeof(int) + 4096;
     const size_t blob_dev_off = round_up(C.off, 256);
+    if (arena_limit && blob_dev_off + blob_bound > arena_limit &&
+        (cand_cap > want_cap || kp_cap > want_cap + want_cap / 2)) {
+        // capacities grown by an earlier, larger batch do not fit this lane's
+        // share: lay this batch out with its own
+        cand_cap = want_cap;
+        kp_cap = want_cap + want_cap / 2;
+        return run(B, outs);
+    }
     rc = ensure_arena(blob_dev_off + blob_bound);
     if (rc) return rc;
     char* A = arena;
@@ -595,6 +741,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     auto AP = [&](size_t off) { return A + off; };
 
     std::vector<BlurJob> hj, hj2, hgj, hsj, vj, vnj, vgj, vsj, cj;
+    const std::pair<int, int> forced = blur_forced();
     int vn_rows = 0, hs_taps4 = 0, vs_taps4 = 0, vs_rows = 0;
 
     auto job_bytes = [](const auto& j, int kind) -> double {
@@ -603,19 +750,26 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         else if constexpr (std::is_same_v<J, LumaJob>) return luma_job_bytes(j);
         else return resize_job_bytes(j, kind);
     };
+    auto job_ops = [](const auto& j, int kind) -> double {
+        using J = std::decay_t<decltype(j)>;
+        if constexpr (std::is_same_v<J, BlurJob>) return blur_job_ops(j, kind);
+        else if constexpr (std::is_same_v<J, LumaJob>) return 0.0;
+        else return resize_job_ops(j, kind);
+    };
     auto push_launch = [&](int kind, auto& jobs, int max_pad, int variant) {
         if (jobs.empty()) return;
         int tiles = 0;
-        double bytes = 0.0;
+        double bytes = 0.0, ops = 0.0;
         for (auto& j : jobs) {
             j.tile_begin = tiles;
             tiles += j.tiles;
             bytes += job_bytes(j, kind);
+            ops += job_ops(j, kind);
         }
         if (tiles == 0) return;
         size_t off = blob_take(jobs.size() * sizeof(jobs[0]));
         memcpy(blob.data() + off, jobs.data(), jobs.size() * sizeof(jobs[0]));
-        launches.push_back({kind, off, (int)jobs.size(), tiles, max_pad, variant, bytes});
+        launches.push_back({kind, off, (int)jobs.size(), tiles, max_pad, variant, bytes, ops});
     };
 
     auto make_blur_pair = [&](const uint8_t* src, int src_pitch, uint8_t* tmp, int tmp_pitch,
@@ -640,30 +794,22 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         J.taps = (const float*)(T + R.off);
         J.ntaps = R.ntaps;
         J.ntaps_pad = R.npad;
+        const std::pair<int, int> form = blur_choice(w, h, R.ntaps, R.npad, n >= kTuneManyJobs, forced);
+        const int n4 = (R.ntaps + 3) & ~3, r = R.ntaps / 2;
         BlurJob H = J;
         H.src = src;
         H.src_pitch = src_pitch;
         H.dst = tmp;
         H.dst_pitch = tmp_pitch;
-        const int px = w >= 384 ? 8 : 2;
-        const int hrow = 64 * px + R.npad + 8;
-        const int n4 = (R.ntaps + 3) & ~3, r = R.ntaps / 2;
-        // small levels and long kernels on narrow levels: one output per lane
-        const bool small_level = (long long)w * h <= 65536 || r >= 24;
-        if (w < 384 && n4 <= kSmallMaxTaps4 && small_level) {
-            H.tiles_x = (w + 63) / 64;
-            H.tiles = H.tiles_x * ((h + 3) / 4);
-            hsj.push_back(H);
-            hs_taps4 = std::max(hs_taps4, n4);
-        } else if ((hrow + 8 + ((hrow + 8) >> (px == 8 ? 3 : 5)) + 1) * 4 * (int)sizeof(float) <=
-                   65536) {  // the 4 staged rows' LDS (padded as k_blur_h<px> pads them)
-            H.tiles_x = (w + 64 * px - 1) / (64 * px);
-            H.tiles = H.tiles_x * ((h + 3) / 4);
-            (px == 8 ? hj : hj2).push_back(H);
-        } else {
-            H.tiles_x = (w + 255) / 256;
-            H.tiles = H.tiles_x * h;
-            hgj.push_back(H);
+        blur_h_tiles(H, form.first);
+        switch (form.first) {
+            case BH_W8: hj.push_back(H); break;
+            case BH_W2: hj2.push_back(H); break;
+            case BH_SMALL:
+                hsj.push_back(H);
+                hs_taps4 = std::max(hs_taps4, n4);
+                break;
+            default: hgj.push_back(H); break;
         }
         BlurJob V = J;
         V.src = tmp;
@@ -673,26 +819,19 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         V.dog = dog;
         V.prev = prev;
         V.dog_pitch = dog_pitch;
-        const int narrow_rows = std::min(h, 16 + 2 * r);
-        if (h <= kSmallMaxRows && n4 <= kSmallMaxTaps4 && (small_level || w < 192 || R.ntaps > 64)) {
-            V.tiles_x = (w + 63) / 64;
-            V.tiles = V.tiles_x * ((h + 3) / 4);
-            vsj.push_back(V);
-            vs_taps4 = std::max(vs_taps4, n4);
-            vs_rows = std::max(vs_rows, std::min(h, 4 + 2 * r));
-        } else if (w >= 192 && R.ntaps <= 64) {
-            V.tiles_x = (w + 255) / 256;
-            V.tiles = V.tiles_x * ((h + 15) / 16);
-            vj.push_back(V);
-        } else if (narrow_rows * 64 <= 65536) {
-            V.tiles_x = (w + 63) / 64;
-            V.tiles = V.tiles_x * ((h + 15) / 16);
-            vnj.push_back(V);
-            vn_rows = std::max(vn_rows, narrow_rows);
-        } else {
-            V.tiles_x = (w + 255) / 256;
-            V.tiles = V.tiles_x * h;
-            vgj.push_back(V);
+        blur_v_tiles(V, form.second);
+        switch (form.second) {
+            case BV_WIDE: vj.push_back(V); break;
+            case BV_NARROW:
+                vnj.push_back(V);
+                vn_rows = std::max(vn_rows, std::min(h, 16 + 2 * r));
+                break;
+            case BV_SMALL:
+                vsj.push_back(V);
+                vs_taps4 = std::max(vs_taps4, n4);
+                vs_rows = std::max(vs_rows, std::min(h, 4 + 2 * r));
+                break;
+            default: vgj.push_back(V); break;
         }
     };
 
@@ -847,6 +986,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
 
     if (blob.size() > blob_bound) return fail(SIFT_ERR_DEVICE, "internal: job blob bound exceeded");
 
+    hipStream_t st = stream;
     // staging: inputs + blob in pinned memory
     size_t in_bytes = 0;
     if (!B.device_input)
@@ -856,7 +996,6 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(stream));  // pinned staging is reused across calls
     memcpy(pinned, blob.data(), blob.size());
-    hipStream_t st = stream;
     HIP_TRY(hipMemcpyAsync(A + blob_dev_off, pinned, blob.size(), hipMemcpyHostToDevice, st));
     if (!B.device_input) {
         size_t po = round_up(blob.size(), 256);
@@ -930,7 +1069,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
                     launch_resize_fused((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st);
                     break;
             }
-            prof_end(L.kind, L.bytes, ev);
+            prof_end(L.kind, L.bytes, ev, L.ops);
         }
         HIP_TRY(hipGetLastError());
         if (B.pyramid_only) return SIFT_OK;
@@ -960,7 +1099,8 @@ int Context::run(const Batch& B, RunOutputs* outs) {
                       B.want_stages ? (float*)AP(candresp_off) : nullptr, st);
         prof_end(K_REFINE, 0.0, ev);
         prof_begin(K_ORIENT, 0.0, &ev);
-        launch_orient(d_oct, d_par, d_cand, d_total, cap, d_ref, d_valid, d_npk, d_ang, st);
+        launch_orient(d_oct, d_par, d_cand, d_total, cap, d_ref, d_valid, d_npk, d_ang,
+                      d_counts + nslots + 2, st);  // kNumXcd queue counters
         prof_end(K_ORIENT, 0.0, ev);
         prof_begin(K_SCAN, 0.0, &ev);
         launch_scan_dev(d_npk, cap, d_total, d_koff, (unsigned*)AP(scantmp2_off), st);
@@ -968,7 +1108,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         prof_begin(K_DESCRIPTOR, 0.0, &ev);
         launch_descriptor(d_oct, d_par, d_cand, d_total, cap, d_ref, d_npk, d_ang, d_koff, kcap, d_kps,
                           d_desc, d_counts + nslots, (int*)AP(wide_off), d_counts + nslots + 1,
-                          (int*)AP(mid_off), st);
+                          (int*)AP(mid_off), d_counts + nslots + 2 + kNumXcd, st);
         prof_end(K_DESCRIPTOR, 0.0, ev);
         prof_begin(K_FINALIZE, 0.0, &ev);
         launch_finalize(d_offs, (const int*)(D + isb_blob), n, d_koff, d_total, cap, d_sum, st);
@@ -1076,6 +1216,74 @@ int Context::run(const Batch& B, RunOutputs* outs) {
 }
 
 }  // namespace sh
+
+// ===========================================================================
+// Tuning table
+// ===========================================================================
+namespace {
+
+// The table file: SIFT_HIP_TUNE_TABLE (a path; "0" = none), else blur_tune.json
+// next to this library.
+std::string tune_table_path() {
+    const char* e = getenv("SIFT_HIP_TUNE_TABLE");
+    if (e) return (e[0] == '0' && e[1] == 0) ? std::string() : std::string(e);
+    Dl_info info{};
+    if (!dladdr((const void*)&tune_table_path, &info) || !info.dli_fname) return std::string();
+    std::string lib(info.dli_fname);
+    const size_t slash = lib.rfind('/');
+    return (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/blur_tune.json";
+}
+
+// Reads the "entries" of the table: arrays [w, h, taps, many, "<h form>", "<v form>"]
+// (every innermost array of the file with those six fields). Unknown forms and
+// malformed arrays are skipped.
+void load_tune_table(std::map<sh::TuneKey, std::pair<int, int>>& out) {
+    const std::string path = tune_table_path();
+    if (path.empty()) return;
+    FILE* f = fopen(path.c_str(), "rb");
+    if (!f) return;
+    std::string text;
+    char buf[4096];
+    size_t k;
+    while ((k = fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, k);
+    fclose(f);
+    const size_t at = text.find("\"entries\"");
+    if (at == std::string::npos) return;
+    size_t i = at;
+    while (i < text.size()) {
+        const size_t open = text.find('[', i);
+        if (open == std::string::npos) break;
+        const size_t close = text.find(']', open + 1);
+        if (close == std::string::npos) break;
+        const size_t inner = text.find('[', open + 1);
+        if (inner != std::string::npos && inner < close) {  // not innermost: descend
+            i = inner;
+            continue;
+        }
+        std::vector<std::string> fields;
+        std::string cur;
+        bool quoted = false;
+        for (size_t p = open + 1; p < close; p++) {
+            const char c = text[p];
+            if (c == '"') quoted = !quoted;
+            else if (c == ',' && !quoted) {
+                fields.push_back(cur);
+                cur.clear();
+            } else if (quoted || (c != ' ' && c != '\n' && c != '\t' && c != '\r')) cur += c;
+        }
+        fields.push_back(cur);
+        if (fields.size() == 6) {
+            const int hf = sh::blur_form_id(fields[4].c_str(), false);
+            const int vf = sh::blur_form_id(fields[5].c_str(), true);
+            const int w = atoi(fields[0].c_str()), h = atoi(fields[1].c_str());
+            const int nt = atoi(fields[2].c_str()), many = atoi(fields[3].c_str()) ? 1 : 0;
+            if (hf >= 0 && vf >= 0 && w > 0 && h > 0 && nt > 0) out[sh::TuneKey{w, h, nt, many}] = {hf, vf};
+        }
+        i = close + 1;
+    }
+}
+
+}  // namespace
 
 // ===========================================================================
 // C ABI
@@ -1187,16 +1395,25 @@ int run_and_collect(Context* C, const Batch& B, sift_result_t* out, sift_result_
         if (out) out[i] = sift_result_t{nullptr, nullptr, 0};
         if (out8) out8[i] = sift_result_u8_t{nullptr, nullptr, 0};
     }
+    // on an error, entries collect() already filled are freed: the caller gets
+    // empty results with the error code
+    auto drop = [&](int rc) {
+        for (int i = 0; i < n; i++) {
+            if (out) sift_result_free(&out[i]);
+            if (out8) sift_result_u8_free(&out8[i]);
+        }
+        return rc;
+    };
     for (int attempt = 0; attempt < 4; attempt++) {
         RunOutputs R;
         int rc = C->run(B, &R);
-        if (rc) return rc;
+        if (rc) return drop(rc);
         bool retry = false;
         rc = collect(C, n, R, out, &retry, out8, kp_conv);
-        if (rc) return rc;
+        if (rc) return drop(rc);
         if (!retry) return SIFT_OK;
     }
-    return fail(SIFT_ERR_NOMEM, "keypoint capacity did not converge");
+    return drop(fail(SIFT_ERR_NOMEM, "keypoint capacity did not converge"));
 }
 }  // namespace sh
 
@@ -1235,6 +1452,7 @@ int sift_hip_create(const sift_params_t* params, int device, sift_hip_ctx** out)
     if (!C) return fail(SIFT_ERR_NOMEM, "context allocation failed");
     C->device = device;
     C->params = *params;
+    load_tune_table(C->tune);
     e = hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete C;
@@ -1488,7 +1706,10 @@ int sift_hip_detect_stages(sift_hip_ctx* ctx, const sift_image_t* image, sift_st
         if (rc) return rc;
         bool retry = false;
         rc = collect(C, 1, R, &out->result, &retry);
-        if (rc) return rc;
+        if (rc) {
+            sift_stages_free(out);
+            return rc;
+        }
         if (retry) continue;
         unsigned nc = 0;
         HIP_TRY(hipMemcpy(&nc, R.d_summary, 4, hipMemcpyDeviceToHost));
@@ -1525,6 +1746,127 @@ int sift_hip_detect_stages(sift_hip_ctx* ctx, const sift_image_t* image, sift_st
         return SIFT_OK;
     }
     return fail(SIFT_ERR_NOMEM, "keypoint capacity did not converge");
+}
+
+int sift_hip_tune_blur(sift_hip_ctx* ctx, uint32_t width, uint32_t height, float sigma, uint32_t jobs,
+                       const char* h_form, const char* v_form, uint32_t reps, double* out_ms,
+                       int* out_forms) {
+    if (!ctx || !out_ms || width == 0 || height == 0 || width > 65536 || height > 65536 || jobs == 0 ||
+        jobs > 1024 || reps == 0 || !(sigma > 0.0f) || sigma > 2000.0f)
+        return fail(SIFT_ERR_INVALID, "invalid argument");
+    sh::Context* C = reinterpret_cast<sh::Context*>(ctx);
+    std::lock_guard<std::mutex> lk(C->mu);
+    (void)hipSetDevice(C->device);
+    const int w = (int)width, h = (int)height;
+    std::vector<float> taps;
+    const int nt = gaussian_taps(sigma, taps);
+    const int npad = (int)round_up((size_t)nt, sh::kTapBlock);
+    taps.resize(npad, 0.0f);
+    const std::pair<int, int> heur = sh::blur_heuristic(w, h, nt, npad);
+    const int hf = h_form && h_form[0] ? sh::blur_form_id(h_form, false) : heur.first;
+    const int vf = v_form && v_form[0] ? sh::blur_form_id(v_form, true) : heur.second;
+    if (hf < 0 || vf < 0) return fail(SIFT_ERR_INVALID, "unknown blur form");
+    if (!sh::blur_h_valid(hf, w, h, nt, npad) || !sh::blur_v_valid(vf, w, h, nt))
+        return fail(SIFT_ERR_INVALID, "blur form not valid for this level");
+    if (out_forms) {
+        out_forms[0] = hf;
+        out_forms[1] = vf;
+    }
+    const size_t toff = C->tables.append(taps.data(), taps.size() * sizeof(float));
+    int rc = C->tables.ensure_uploaded(C->stream);
+    if (rc) return rc;
+    const int pitch = (int)round_up((size_t)w, 64);
+    const size_t level = (size_t)pitch * h + 64;
+    sh::Carver L;
+    const size_t src_off = L.take(level);
+    std::vector<size_t> tmp_off(jobs), dst_off(jobs);
+    for (uint32_t j = 0; j < jobs; j++) {
+        tmp_off[j] = L.take(level);
+        dst_off[j] = L.take(level);
+    }
+    const size_t hj_off = L.take(jobs * sizeof(sh::BlurJob)), vj_off = L.take(jobs * sizeof(sh::BlurJob));
+    if ((rc = C->ensure_arena(L.off))) return rc;
+    if ((rc = C->ensure_pinned(2 * jobs * sizeof(sh::BlurJob) + 256))) return rc;
+    char* A = C->arena;
+    std::vector<sh::BlurJob> hjobs(jobs), vjobs(jobs);
+    int htiles = 0, vtiles = 0;
+    for (uint32_t j = 0; j < jobs; j++) {
+        sh::BlurJob J{};
+        J.w = w;
+        J.h = h;
+        J.taps = (const float*)(C->tables.dev + toff);
+        J.ntaps = nt;
+        J.ntaps_pad = npad;
+        J.src = (const uint8_t*)A + src_off;
+        J.src_pitch = pitch;
+        J.dst = (uint8_t*)A + tmp_off[j];
+        J.dst_pitch = pitch;
+        sh::blur_h_tiles(J, hf);
+        J.tile_begin = htiles;
+        htiles += J.tiles;
+        hjobs[j] = J;
+        J.src = (const uint8_t*)A + tmp_off[j];
+        J.dst = (uint8_t*)A + dst_off[j];
+        sh::blur_v_tiles(J, vf);
+        J.tile_begin = vtiles;
+        vtiles += J.tiles;
+        vjobs[j] = J;
+    }
+    HIP_TRY(hipStreamSynchronize(C->stream));  // pinned staging is reused across calls
+    memcpy(C->pinned, hjobs.data(), jobs * sizeof(sh::BlurJob));
+    memcpy(C->pinned + jobs * sizeof(sh::BlurJob), vjobs.data(), jobs * sizeof(sh::BlurJob));
+    HIP_TRY(hipMemcpyAsync(A + hj_off, C->pinned, jobs * sizeof(sh::BlurJob), hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemcpyAsync(A + vj_off, C->pinned + jobs * sizeof(sh::BlurJob), jobs * sizeof(sh::BlurJob),
+                           hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemsetAsync(A + src_off, 0x6b, level, C->stream));  // any content: the work is data-independent
+    const int n4 = (nt + 3) & ~3, r = nt / 2;
+    const sh::BlurJob* dh = (const sh::BlurJob*)(A + hj_off);
+    const sh::BlurJob* dv = (const sh::BlurJob*)(A + vj_off);
+    auto launch_h = [&]() {
+        switch (hf) {
+            case sh::BH_W8: sh::launch_blur_h(dh, (int)jobs, htiles, npad, 8, C->stream); break;
+            case sh::BH_W2: sh::launch_blur_h(dh, (int)jobs, htiles, npad, 2, C->stream); break;
+            case sh::BH_SMALL: sh::launch_blur_h_small(dh, (int)jobs, htiles, n4, C->stream); break;
+            default: sh::launch_blur_h_generic(dh, (int)jobs, htiles, C->stream); break;
+        }
+    };
+    auto launch_v = [&]() {
+        switch (vf) {
+            case sh::BV_WIDE: sh::launch_blur_v(dv, (int)jobs, vtiles, 0, C->stream); break;
+            case sh::BV_NARROW: sh::launch_blur_v(dv, (int)jobs, vtiles, std::min(h, 16 + 2 * r), C->stream); break;
+            case sh::BV_SMALL:
+                sh::launch_blur_v_small(dv, (int)jobs, vtiles, n4, std::min(h, 4 + 2 * r), C->stream);
+                break;
+            default: sh::launch_blur_v_generic(dv, (int)jobs, vtiles, C->stream); break;
+        }
+    };
+    std::vector<hipEvent_t> ev(3 * (reps + 2), nullptr);
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    for (uint32_t k = 0; k < reps + 2; k++) {  // the first two are warm-up
+        HIP_TRY(hipEventRecord(ev[3 * k], C->stream));
+        launch_h();
+        HIP_TRY(hipEventRecord(ev[3 * k + 1], C->stream));
+        launch_v();
+        HIP_TRY(hipEventRecord(ev[3 * k + 2], C->stream));
+    }
+    const hipError_t le = hipGetLastError();
+    const hipError_t se = hipStreamSynchronize(C->stream);
+    std::vector<float> th, tv;
+    for (uint32_t k = 2; k < reps + 2 && le == hipSuccess && se == hipSuccess; k++) {
+        float a = 0.0f, b = 0.0f;
+        (void)hipEventElapsedTime(&a, ev[3 * k], ev[3 * k + 1]);
+        (void)hipEventElapsedTime(&b, ev[3 * k + 1], ev[3 * k + 2]);
+        th.push_back(a);
+        tv.push_back(b);
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (le != hipSuccess) return fail(SIFT_ERR_DEVICE, std::string("launch failed: ") + hipGetErrorString(le));
+    if (se != hipSuccess) return fail(SIFT_ERR_DEVICE, std::string("synchronize failed: ") + hipGetErrorString(se));
+    std::sort(th.begin(), th.end());
+    std::sort(tv.begin(), tv.end());
+    out_ms[0] = th[th.size() / 2];
+    out_ms[1] = tv[tv.size() / 2];
+    return SIFT_OK;
 }
 
 int sift_hip_profile(sift_hip_ctx* ctx, int enable) {
@@ -1568,6 +1910,7 @@ int sift_hip_profile_read(sift_hip_ctx* ctx, sift_kernel_stat_t* out, size_t cap
             out[k].launches = kv.second.launches;
             out[k].total_ms = kv.second.total_ms;
             out[k].bytes = kv.second.bytes;
+            out[k].ops = kv.second.ops;
         }
         k++;
     }

@@ -13,6 +13,8 @@
 // extremum and wide-window kernels), which is bit-identical to the reference's
 // (G_{s+1} as f32 / 255.0) - (G_s as f32 / 255.0) (src/sift.rs:97,120).
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "sift_internal.h"
 
@@ -669,6 +671,60 @@ __global__ __launch_bounds__(kThreads) void k_refine(const OctaveDesc* __restric
 // each for the two [1,1,1]/3 smoothing passes, the max and the peak test; a
 // ballot ranks the peaks in bin order.
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Work queue of the keypoint kernels. With a queue, a launch has as many
+// work-groups as fit on the GPU at once, and each takes chunks of kChunk
+// consecutive keypoints until the count is reached. Work-group b uses counter
+// b % kNumXcd (the XCD the dispatcher places it on) of kNumXcd counters, zeroed
+// by a fill kernel earlier on the stream; counter x's chunk c holds keypoints
+// [(c * kNumXcd + x) * kChunk, + kChunk). One counter taking every item
+// serialises the atomics of the whole GPU on one address (measured: 10x the
+// kernel's time); eight counters and chunks make them rare. Without a queue,
+// work-group b takes keypoints b, b + grid, ...
+// Thread 0 takes the next chunk at the start of a chunk's last item, into the
+// slot the work-group is not reading, and everyone reads it after the barrier
+// that ends that item: each slot is written before a barrier, read after it,
+// and not written again until every thread has read it. All threads of a
+// work-group thus see the same item sequence, every item's early exits
+// (`continue`) still pass the barrier that ends it, and the loop test is
+// uniform.
+// ---------------------------------------------------------------------------
+template <int kChunk>
+struct WorkQueue {
+    unsigned* heads;  // kNumXcd counters; null: static grid-stride assignment
+    int slot_it, left, x;
+
+    __device__ __forceinline__ int chunk_first(int c) const { return (c * kNumXcd + x) * kChunk; }
+    __device__ __forceinline__ int first(int* slots) {
+        slot_it = 0;
+        left = kChunk - 1;
+        x = (int)(blockIdx.x % kNumXcd);
+        if (!heads) return (int)blockIdx.x;
+        if (threadIdx.x == 0) slots[0] = (int)atomicAdd(&heads[x], 1u);
+        __syncthreads();
+        return chunk_first(slots[0]);
+    }
+    // at the start of an item: thread 0 takes the next chunk during the last
+    // item of this one
+    __device__ __forceinline__ void prefetch(int* slots) {
+        if (heads && left == 0 && threadIdx.x == 0)
+            slots[(slot_it + 1) & 1] = (int)atomicAdd(&heads[x], 1u);
+    }
+    __device__ __forceinline__ int next(int* slots, int cur) {
+        if (!heads) return cur + (int)gridDim.x;
+        __syncthreads();
+        if (left > 0) {
+            left--;
+            return cur + 1;
+        }
+        slot_it++;
+        left = kChunk - 1;
+        return chunk_first(slots[slot_it & 1]);
+    }
+};
+constexpr int kOrientChunk = 8;
+constexpr int kDescChunk = 2;
+
 constexpr int kOriMaxR = 16;
 constexpr int kOriRows = 2 * kOriMaxR + 5;  // staged rows
 constexpr int kOS = 2 * kOriMaxR + 8;       // staged row stride (bytes): 4 * ceil((3 + rows + 3) / 4)
@@ -680,7 +736,8 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                                                const Keypoint* __restrict__ ref,
                                                const int* __restrict__ valid,
                                                unsigned* __restrict__ npeaks,
-                                               float* __restrict__ angles) {
+                                               float* __restrict__ angles,
+                                               unsigned* __restrict__ queue) {
     // 4 histogram copies (lane & 3), the staged window and the q table: 3.9 KB of
     // LDS, so the 8 waves per SIMD that 40 VGPRs allow fit (32 work-groups of one
     // wave per CU; 21 with the window staged as f32)
@@ -693,7 +750,10 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
     __shared__ int ocx[2 * kOriMaxR + 1], ocy[2 * kOriMaxR + 1];
     const int lane = threadIdx.x;
     const int n = (int)min(*total, (unsigned)cap);
-    for (int c = blockIdx.x; c < n; c += gridDim.x) {
+    __shared__ int qslot[2];
+    WorkQueue<kOrientChunk> Q{queue, 0, 0, 0};
+    for (int c = Q.first(qslot); c < n; c = Q.next(qslot, c)) {
+        Q.prefetch(qslot);
         if (!valid[c]) {
             if (lane == 0) npeaks[c] = 0;
             continue;
@@ -917,9 +977,12 @@ struct DescDims {
     // staged patch row stride (bytes): the aligned words of a row, up to 3 + rows + 3 bytes
     static constexpr int PS = (2 * MR + 12) / 4 * 4;
 };
-constexpr int kCellWords = 10;                  // one histogram cell: 8 angle bins + pair padding
-constexpr int kTableWords = 36 * kCellWords;    // 6 x 6 padded cells
-constexpr int kHistWords = 2 * kTableWords;     // one copy: even-pair table + odd-pair table
+// One histogram cell: the even-pair table (bins 0..7 in words 0..7) and the
+// odd-pair table (bin b in word 9 + b, bin 0 again in word 17); 18 words keep
+// every cell, and with it every bin pair, 8-byte aligned.
+constexpr int kCellWords = 18;
+constexpr int kOddBase = 9;                     // word of bin 0 in the odd-pair table
+constexpr int kHistWords = 36 * kCellWords;     // one copy: 6 x 6 padded cells
 constexpr int kHistCopies = 2;                  // even and odd lanes scatter into separate copies
 constexpr int kMaxCopySamples = 2048;           // per-copy sample bound that keeps u32 sums exact
 
@@ -929,9 +992,9 @@ constexpr int kMaxCopySamples = 2048;           // per-copy sample bound that ke
 // reference drops neighbours outside the 4 x 4 grid, here they land in border
 // cells that are never read, so there is no clamping and the four spatial
 // neighbours are constant offsets. The two angle bins (a, a+1 mod 8) of each
-// spatial neighbour go in ONE 64-bit LDS add: the even-pair table holds bins
-// 0..7 in slots 0..7 (pairs (0,1), (2,3), ... are 8-byte aligned); the
-// odd-pair table holds bin b in slot b+1 and bin 0 again in slot 9, so pairs
+// spatial neighbour go in ONE 64-bit LDS add: a cell's even-pair table holds
+// bins 0..7 in its words 0..7 (pairs (0,1), (2,3), ... are 8-byte aligned); its
+// odd-pair table holds bin b in word 9 + b and bin 0 again in word 17, so pairs
 // (1,2), (3,4), (5,6), (7,0) are aligned too. Every bin sum of a copy stays
 // below 2^32 (at most kMaxCopySamples samples of at most sqrt(2) * fix + 1/2
 // each), so the low word never carries into the high word.
@@ -954,7 +1017,7 @@ __device__ __forceinline__ unsigned long long* desc_bins(unsigned* hcopy, float 
     tt[2] = t0 * dy;
     tt[3] = t1 * dy;
     return reinterpret_cast<unsigned long long*>(hcopy + 7 * kCellWords +
-                                                 (cw + ai + __umul24(ai & 1u, kTableWords + 1u)));
+                                                 (cw + ai + __umul24(ai & 1u, (unsigned)kOddBase)));
 }
 
 // Adds four pairs of fixed-point sums (a[2k], a[2k+1]) at the spatial
@@ -1068,9 +1131,9 @@ __device__ __forceinline__ void desc_finish(const unsigned* hist, float fix, int
 #pragma unroll
         for (int cp = 0; cp < kHistCopies; cp++) {
             const unsigned* t = hist + cp * kHistWords + cell * kCellWords;
-            sum += t[a];                    // even-pair table, slot a
-            sum += t[kTableWords + a + 1];  // odd-pair table, slot a + 1
-            if (a == 0) sum += t[kTableWords + 9];
+            sum += t[a];                // even-pair table
+            sum += t[kOddBase + a];     // odd-pair table
+            if (a == 0) sum += t[kOddBase + 8];
         }
         v[e] = (float)((double)sum / (double)fix);
     }
@@ -1126,33 +1189,47 @@ __device__ __forceinline__ void desc_finish(const unsigned* hist, float fix, int
 // ---------------------------------------------------------------------------
 template <int MR, bool kFromList>
 __global__ __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 7 : 6))) void k_descriptor(
+__attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 8 : 6))) void k_descriptor(
     const OctaveDesc* __restrict__ oct, const ImageParams* __restrict__ params,
     const Cand* __restrict__ cand, const unsigned* __restrict__ total, int cap,
     const Keypoint* __restrict__ ref, const unsigned* __restrict__ npeaks,
     const float* __restrict__ angles, const unsigned* __restrict__ koff, int kcap,
     Keypoint* __restrict__ kps_out, float* __restrict__ desc_out,
     const unsigned* __restrict__ in_count, const int* __restrict__ in_list,
-    unsigned* __restrict__ out_count, int* __restrict__ out_list) {
+    unsigned* __restrict__ out_count, int* __restrict__ out_list, unsigned* __restrict__ queue) {
     using D = DescDims<MR>;
     constexpr int kGS = D::GS, kPatchRows = D::PatchRows, kPS = D::PS;
     static_assert(kHistCopies * kHistWords * 4 >= kPatchRows * kPS + 4, "patch shares the histograms");
-    // the u8 patch is dead once the grid is built; the histograms live after that
+    // The u8 patch (and, when they fit beside it, the grid's column and row
+    // terms) are dead once the grid is built; the histograms live after that.
+    constexpr int kPatchBytes = (kPatchRows * kPS + 15) / 16 * 16;
+    constexpr bool kTermsInPatch = kPatchBytes + 2 * kGS * (int)sizeof(float2) <= kHistCopies * kHistWords * 4;
     __shared__ __align__(16) unsigned pu[kHistCopies * kHistWords];
     __shared__ float grid[kGS * kGS];
     __shared__ float exs[2 * MR + 1];
-    __shared__ float4 rrec[64];  // per row: first and last column (int bits), sin * py, cos * py
-    __shared__ float rwy[64];
+    __shared__ int rspan[64];  // per row: first and last column of its samples (16 bits each)
     __shared__ int rpre[65];
     __shared__ int seg_len;
-    __shared__ float2 gcol[kGS], grw[kGS];  // grid columns / rows: texel offset, fraction
+    __shared__ float2 terms_own[kTermsInPatch ? 1 : 2 * kGS];
     __shared__ float pang[kMaxPeaks], pcos[kMaxPeaks], psin[kMaxPeaks];
     uint8_t* const patch = reinterpret_cast<uint8_t*>(pu);
     unsigned* const hist = pu;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    // grid columns / rows: texel offset, fraction
+    float2* const gcol = kTermsInPatch ? reinterpret_cast<float2*>(patch + kPatchBytes) : terms_own;
+    float2* const grw = gcol + kGS;
     // all candidates, or the ones an instantiation for smaller windows listed
     const int n = (int)min(kFromList ? *in_count : *total, (unsigned)cap);
-    for (int idx = blockIdx.x; idx < n; idx += gridDim.x) {
+    __shared__ int qslot[2];
+    WorkQueue<kDescChunk> Q{queue, 0, 0, 0};
+    for (int idx = Q.first(qslot); idx < n; idx = Q.next(qslot, idx)) {
+        Q.prefetch(qslot);
+        // the thread index passes through an empty asm in every iteration, so
+        // the many cheap values derived from it are formed where they are used
+        // instead of being hoisted out of this loop and kept live across it
+        // (spilled to scratch at 7 waves per SIMD)
+        int tid = (int)threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int wave = tid >> 6, lane = tid & 63;
         const int c = kFromList ? in_list[idx] : idx;
         const DescGeom G = desc_geom(c, oct, params, cand, ref, npeaks, koff);
         if (G.np == 0) continue;
@@ -1183,6 +1260,7 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 7 : 6))) void k_descripto
 #pragma unroll
                 for (int k = 0; k < 5; k++) {
                     const int i = tid + k * kThreads;
+                    wv[k] = 0u;  // defined on every path: nothing carried across keypoints
                     if (i < tot) {
                         const int v = (int)(((float)i + 0.5f) * inv_nw);
                         const int wd = i - v * nw;
@@ -1268,9 +1346,7 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 7 : 6))) void k_descripto
                     const int b = min((int)floorf(hif) + R + 1, bx1);
                     lo = a;
                     cnt = max(b - a + 1, 0);
-                    rrec[tid] = make_float4(__int_as_float(lo), __int_as_float(lo + cnt - 1),
-                                            sa * py * inv_bw, ca * py * inv_bw);
-                    rwy[tid] = exs[cy] * (kDescFix / 255.0f);
+                    rspan[tid] = lo | ((lo + cnt - 1) << 16);  // columns < 64, last >= lo - 1
                 }
                 // Each row is cut into segments of at most `per` consecutive samples,
                 // one segment per thread: per is the smallest value >= T / 256 that
@@ -1300,10 +1376,15 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 7 : 6))) void k_descripto
                     if (rpre[mid] <= sg) r0 = mid;
                     else r1 = mid;
                 }
-                const float4 rr = rrec[r0];
-                int cx = __float_as_int(rr.x) + (sg - rpre[r0]) * per;
-                const int len = min(per, __float_as_int(rr.y) - cx + 1);
-                const float wyf = rwy[r0];
+                const int span = rspan[r0];
+                int cx = (span & 0xffff) + (sg - rpre[r0]) * per;
+                const int len = min(per, (span >> 16) - cx + 1);
+                // the row's terms (src/sift.rs:967-968 at px = 0, and the row's
+                // Gaussian factor), formed as the row-interval pass forms them
+                const int cy = by0 + r0;
+                const float py = (float)(cy - R);
+                const float rz = sa * py * inv_bw, rw = ca * py * inv_bw;
+                const float wyf = exs[cy] * (kDescFix / 255.0f);
                 // grid point (cx, cy) of sample (cx, cy) -- the up-left neighbour -- so that
                 // all four gradient neighbours sit at non-negative offsets
                 const float* grow = grid + (by0 + r0) * kGS;
@@ -1316,8 +1397,8 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 7 : 6))) void k_descripto
                 // per run
                 unsigned long long* acc_base = nullptr;
                 float acc[8];
-                float u = __builtin_fmaf(cai, (float)(cx - R), rr.z);
-                float v = __builtin_fmaf(msai, (float)(cx - R), rr.w);
+                float u = __builtin_fmaf(cai, (float)(cx - R), rz);
+                float v = __builtin_fmaf(msai, (float)(cx - R), rw);
                 for (int i = 0; i < len; i++, cx++, u += cai, v += msai) {
                     // window: bin coordinates u + 1.5, v + 1.5 in (-1, 4) (src/sift.rs:969-977)
                     if (!(fabsf(u) < 2.5f && fabsf(v) < 2.5f)) continue;
@@ -1347,9 +1428,15 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 7 : 6))) void k_descripto
                 if (acc_base) desc_add_pairs<1>(acc_base, acc);
             }
             __syncthreads();
-            if (tid < 64)
-                desc_finish(hist, kDescFix, tid, G.base, angle, G.k0 + (unsigned)pk, kcap, kps_out,
+            if (tid < 64) {
+                // the keypoint and its output slot are read again here rather than
+                // kept in registers through the peak loop (the index goes through an
+                // empty asm so the compiler does not reuse the first reads)
+                int cr = c;
+                asm volatile("" : "+v"(cr));
+                desc_finish(hist, kDescFix, tid, ref[cr], angle, koff[cr] + (unsigned)pk, kcap, kps_out,
                             desc_out);
+            }
             __syncthreads();
         }
     }
@@ -1461,8 +1548,8 @@ __global__ __launch_bounds__(256) void k_desc_u8(const float4* __restrict__ desc
 
 void launch_extrema(const OctaveDesc* d_oct, int noct, int total_rows, const ImageParams* d_params,
                     unsigned long long* d_bits, unsigned* d_counts, int nslots, hipStream_t s) {
-    // counts[nslots], counts[nslots + 1]: the descriptor's list counts
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, d_counts + nslots, 2, 0u);
+    // the counters after the row-slot counts (the descriptor's lists, the work queues)
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, d_counts + nslots, kCountExtra, 0u);
     if (total_rows <= 0) return;
     hipLaunchKernelGGL(k_extrema, dim3(total_rows), dim3(kThreads), 0, s, d_oct, noct, d_params,
                        d_bits, d_counts);
@@ -1473,8 +1560,9 @@ bool extrema_tile_supported(int S) { return S >= 1 && S <= 6; }
 void launch_extrema_tile(int S, int strip_rows, const OctaveDesc* d_oct, int noct, int total_tiles,
                          const ImageParams* d_params, unsigned long long* d_bits, unsigned* d_counts,
                          int nslots, hipStream_t s) {
-    // counts[nslots], counts[nslots + 1]: the descriptor's list counts
-    hipLaunchKernelGGL(k_fill_u32, dim3((nslots + 2 + 255) / 256), dim3(256), 0, s, d_counts, nslots + 2, 0u);
+    // the row-slot counts and the counters after them (the descriptor's lists, the work queues)
+    hipLaunchKernelGGL(k_fill_u32, dim3((nslots + kCountExtra + 255) / 256), dim3(256), 0, s, d_counts,
+                       nslots + kCountExtra, 0u);
     if (total_tiles <= 0) return;
     switch (S) {
 #define SH_EXT(SS) \
@@ -1512,6 +1600,36 @@ static int grid_for(int cap, int per_block, int max_blocks) {
     return g < 1 ? 1 : (g > max_blocks ? max_blocks : g);
 }
 
+// Work-groups of `kernel` that fit on the current device at once (its
+// occupancy times the CU count), computed once per kernel.
+static int resident_groups(const void* kernel, int block) {
+    static std::mutex mu;
+    static std::map<const void*, int> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(kernel);
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        per_cu <= 0 || cus <= 0) {
+        per_cu = 4;
+        cus = 256;
+    }
+    const int g = per_cu * cus;
+    cache[kernel] = g;
+    return g;
+}
+
+// The keypoint kernels take their keypoints by a static grid-stride assignment;
+// SIFT_HIP_QUEUE=1 (read on every launch) gives them the device work queue
+// instead (the same results; on 64 x 1920x1080 frames it was 45% slower for
+// k_orient and even for k_descriptor).
+static bool queue_enabled() {
+    const char* e = getenv("SIFT_HIP_QUEUE");
+    return e && e[0] == '1';
+}
+
 void launch_refine(const OctaveDesc* d_oct, const ImageParams* d_params, const Cand* d_cand,
                    const unsigned* d_total, int cap, Keypoint* d_ref, int* d_valid,
                    float* d_cand_resp, hipStream_t s) {
@@ -1521,29 +1639,41 @@ void launch_refine(const OctaveDesc* d_oct, const ImageParams* d_params, const C
 
 void launch_orient(const OctaveDesc* d_oct, const ImageParams* d_params, const Cand* d_cand,
                    const unsigned* d_total, int cap, const Keypoint* d_ref, const int* d_valid,
-                   unsigned* d_npeaks, float* d_angles, hipStream_t s) {
-    // grid-stride over the candidates with many more workgroups than fit at once:
-    // each workgroup takes only a few keypoints, so the launch ends evenly
-    hipLaunchKernelGGL(k_orient, dim3(grid_for(cap, 1, 65536)), dim3(64), 0, s, d_oct, d_params,
-                       d_cand, d_total, cap, d_ref, d_valid, d_npeaks, d_angles);
+                   unsigned* d_npeaks, float* d_angles, unsigned* d_queue, hipStream_t s) {
+    // work queue: as many work-groups as fit, each taking keypoints until none
+    // are left; or grid-stride over the candidates with many more work-groups
+    // than fit at once (each takes only a few keypoints, so the launch ends evenly)
+    const bool q = queue_enabled();
+    const int grid = q ? std::min(resident_groups((const void*)k_orient, 64), grid_for(cap, 1, 65536))
+                       : grid_for(cap, 1, 65536);
+    // (d_queue: kNumXcd counters)
+    hipLaunchKernelGGL(k_orient, dim3(grid), dim3(64), 0, s, d_oct, d_params, d_cand, d_total, cap,
+                       d_ref, d_valid, d_npeaks, d_angles, q ? d_queue : nullptr);
 }
 
 void launch_descriptor(const OctaveDesc* d_oct, const ImageParams* d_params, const Cand* d_cand,
                        const unsigned* d_total, int cap, const Keypoint* d_ref,
                        const unsigned* d_npeaks, const float* d_angles, const unsigned* d_koff,
                        int kcap, Keypoint* d_kps, float* d_desc, unsigned* d_wide_count,
-                       int* d_wide_list, unsigned* d_mid_count, int* d_mid_list, hipStream_t s) {
-    // as k_orient: few keypoints per workgroup, so the launch ends evenly. Windows up
-    // to kDescSmallR run at 7 work-groups per CU and list the few others for the
+                       int* d_wide_list, unsigned* d_mid_count, int* d_mid_list, unsigned* d_queues,
+                       hipStream_t s) {
+    // as k_orient: a work queue, or few keypoints per work-group. Windows up to
+    // kDescSmallR run at 8 work-groups per CU and list the few others for the
     // kDescMaxR instantiation (6 per CU), which lists still wider ones for
-    // k_descriptor_wide.
-    const int grid = grid_for(cap, 1, 49152);
-    hipLaunchKernelGGL((k_descriptor<kDescSmallR, false>), dim3(grid), dim3(kThreads), 0, s, d_oct,
+    // k_descriptor_wide. d_queues: the two instantiations' queue counters
+    // (kNumXcd each).
+    const bool q = queue_enabled();
+    const void* k28 = (const void*)k_descriptor<kDescSmallR, false>;
+    const void* k31 = (const void*)k_descriptor<kDescMaxR, true>;
+    const int g28 = q ? std::min(resident_groups(k28, kThreads), grid_for(cap, 1, 49152)) : grid_for(cap, 1, 49152);
+    const int g31 = q ? std::min(resident_groups(k31, kThreads), grid_for(cap, 1, 49152)) : grid_for(cap, 1, 49152);
+    hipLaunchKernelGGL((k_descriptor<kDescSmallR, false>), dim3(g28), dim3(kThreads), 0, s, d_oct,
                        d_params, d_cand, d_total, cap, d_ref, d_npeaks, d_angles, d_koff, kcap,
-                       d_kps, d_desc, nullptr, nullptr, d_mid_count, d_mid_list);
-    hipLaunchKernelGGL((k_descriptor<kDescMaxR, true>), dim3(grid), dim3(kThreads), 0, s, d_oct,
+                       d_kps, d_desc, nullptr, nullptr, d_mid_count, d_mid_list, q ? d_queues : nullptr);
+    hipLaunchKernelGGL((k_descriptor<kDescMaxR, true>), dim3(g31), dim3(kThreads), 0, s, d_oct,
                        d_params, d_cand, d_total, cap, d_ref, d_npeaks, d_angles, d_koff, kcap,
-                       d_kps, d_desc, d_mid_count, d_mid_list, d_wide_count, d_wide_list);
+                       d_kps, d_desc, d_mid_count, d_mid_list, d_wide_count, d_wide_list,
+                       q ? d_queues + kNumXcd : nullptr);
     hipLaunchKernelGGL(k_descriptor_wide, dim3(256), dim3(kThreads), 0, s, d_oct, d_params, d_cand,
                        d_total, cap, d_ref, d_npeaks, d_angles, d_koff, kcap, d_kps, d_desc,
                        d_wide_count, d_wide_list);

@@ -101,15 +101,28 @@ struct RunOutputs {
 
 struct Server;
 
+// Blur pass forms (context.hip) and the tuning table's key: a level's width,
+// height and tap count, and whether its launch carries many jobs (a batch of
+// at least kTuneManyJobs frames) or few.
+enum BlurHForm { BH_W8 = 0, BH_W2, BH_SMALL, BH_GENERIC, BH_COUNT };
+enum BlurVForm { BV_WIDE = 0, BV_NARROW, BV_SMALL, BV_GENERIC, BV_COUNT };
+constexpr int kTuneManyJobs = 8;
+struct TuneKey {
+    int w, h, ntaps, many;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(w, h, ntaps, many) < std::tie(o.w, o.h, o.ntaps, o.many);
+    }
+};
+
 // Per-kernel event timing (sift_hip_profile).
 struct ProfPending {
     int kernel;
-    double bytes;
+    double bytes, ops;
     hipEvent_t a, b;
 };
 struct ProfStat {
     uint64_t launches = 0;
-    double total_ms = 0.0, bytes = 0.0;
+    double total_ms = 0.0, bytes = 0.0, ops = 0.0;
 };
 
 // A captured launch sequence of one batch layout (see Context::run).
@@ -131,13 +144,24 @@ struct Context {
 
     char* arena = nullptr;
     size_t arena_cap = 0;
+    // Growth rule of the arena: at least arena_floor bytes per allocation, and
+    // never more than arena_limit (0 = no limit; a server lane's share of the HBM
+    // budget). A call whose layout needs more than the limit fails with
+    // SIFT_ERR_NOMEM.
+    size_t arena_floor = (size_t)64 << 20;
+    size_t arena_limit = 0;
     char* pinned = nullptr;
     size_t pinned_cap = 0;
     char* pinned_out = nullptr;  // device-to-host staging for results
     size_t pinned_out_cap = 0;
     size_t cand_cap = 0, kp_cap = 0;
 
-    Server* server = nullptr;  // owned; created by sift_hip_server_start
+    // The server of this context (owned; created by sift_hip_server_start).
+    // server_mu guards the pointer and the server's count of callers inside
+    // submit / poll, so that sift_hip_server_stop deletes it only after they left.
+    std::mutex server_mu;
+    std::condition_variable server_cv;
+    Server* server = nullptr;
 
     // Launch sequences replayed as HIP graphs, keyed by everything the kernel
     // arguments depend on (arena base, layout offsets, launch shapes, capacities).
@@ -147,6 +171,7 @@ struct Context {
     uint64_t graph_clock = 0;
     void clear_graphs();
 
+
     bool profiling = false;
     int prof_only = -1;  // kernel id whose launches are timed, -1 = all
     std::vector<ProfPending> prof_pending;
@@ -154,8 +179,14 @@ struct Context {
     std::map<int, ProfStat> prof_stats;
     hipEvent_t prof_event();
     void prof_begin(int kernel, double bytes, hipEvent_t* a);
-    void prof_end(int kernel, double bytes, hipEvent_t a);
+    void prof_end(int kernel, double bytes, hipEvent_t a, double ops = 0.0);
     void prof_resolve();  // call after the stream is idle
+
+    // Blur forms chosen by the tuning table (w, h, taps, many) -> (H form, V form),
+    // loaded at creation; shapes it does not list take the built-in choice.
+    std::map<TuneKey, std::pair<int, int>> tune;
+    std::pair<int, int> blur_choice(int w, int h, int ntaps, int npad, bool many,
+                                    std::pair<int, int> forced) const;
 
     const ParamTables& tables_for(const sift_params_t& p);
     const LanczosRef& lanczos_for(int in, int out);
@@ -167,6 +198,13 @@ struct Context {
     // for table uploads); outputs stay in the arena until the next run.
     int run(const Batch& B, RunOutputs* outs);
 };
+
+// Sets the calling thread's sift_hip_last_error message; returns code.
+int set_error(int code, const char* msg);
+
+// Device bytes one frame takes in a batch at the initial capacities: an upper
+// bound of what Context::run carves for it (server admission).
+size_t frame_device_bytes(uint32_t w, uint32_t h, const sift_params_t& p, int bpp);
 
 // Runs a host-input batch and copies the results out (f32 descriptors into
 // `out`, or u8 descriptors into `out8` when the batch has desc_u8 set), growing

@@ -153,14 +153,18 @@ void launch_compact(const OctaveDesc* d_oct, int noct, int nslots, const unsigne
 void launch_refine(const OctaveDesc* d_oct, const ImageParams* d_params, const Cand* d_cand,
                    const unsigned* d_total, int cap, Keypoint* d_ref, int* d_valid,
                    float* d_cand_resp, hipStream_t s);
+// d_queue / d_queues: zeroed work-queue counters (kNumXcd for k_orient, kNumXcd
+// for each of the descriptor's two instantiations), among the kCountExtra after
+// the row slots.
 void launch_orient(const OctaveDesc* d_oct, const ImageParams* d_params, const Cand* d_cand,
                    const unsigned* d_total, int cap, const Keypoint* d_ref, const int* d_valid,
-                   unsigned* d_npeaks, float* d_angles, hipStream_t s);
+                   unsigned* d_npeaks, float* d_angles, unsigned* d_queue, hipStream_t s);
 void launch_descriptor(const OctaveDesc* d_oct, const ImageParams* d_params, const Cand* d_cand,
                        const unsigned* d_total, int cap, const Keypoint* d_ref,
                        const unsigned* d_npeaks, const float* d_angles, const unsigned* d_koff,
                        int kcap, Keypoint* d_kps, float* d_desc, unsigned* d_wide_count,
-                       int* d_wide_list, unsigned* d_mid_count, int* d_mid_list, hipStream_t s);
+                       int* d_wide_list, unsigned* d_mid_count, int* d_mid_list, unsigned* d_queues,
+                       hipStream_t s);
 // Writes summary[0] = candidates, summary[1] = keypoints, summary[2..2+nimg] =
 // per-image keypoint offsets (nimg+1 entries).
 void launch_finalize(const unsigned* d_slot_offsets, const int* d_img_slot_begin, int nimg,
@@ -176,6 +180,10 @@ constexpr int kMaxHPad = 3576;
 constexpr int kSmallMaxTaps4 = 2800;
 constexpr int kSmallMaxRows = 128;
 constexpr int kScanTileElems = 4096;
+// Counters after the extremum row-slot counts, zeroed with them: the
+// descriptor's two lists, then the work-queue counters (kNumXcd each) of
+// k_orient and of the two descriptor instantiations.
+constexpr int kCountExtra = 2 + 3 * kNumXcd;
 // Lanczos3 resize tiles: output rows per work-group of the horizontal and fused
 // kernels, and the input columns the fused kernel stages per tile.
 constexpr int kResizeRows = 8;

@@ -74,3 +74,42 @@ def batch_shapes(n=64, seed=77):
     rng = np.random.default_rng(seed)
     idx = rng.integers(0, len(BATCH_SHAPES), size=n)
     return [BATCH_SHAPES[i] for i in idx]
+
+
+# The served workload: an open-loop Poisson stream with sizes in a 40/30/20/10% mix;
+# request j's frame is generic_image(w, h, seed=100000 + j), octaves floor(log2 min) - 3
+# (8 for 3840x2160).
+SERVED_SIZES = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+SERVED_MIX = [0.4, 0.3, 0.2, 0.1]
+
+
+def served_stream(n, rate=1000.0, seed=7):
+    """(arrival times in s, size index per request) of requests 0..n-1."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    which = rng.choice(len(SERVED_SIZES), size=n, p=SERVED_MIX)
+    return arrivals, which
+
+
+def served_frames(which, per_size):
+    """A rotating pool of distinct frames for a served stream: the first
+    per_size[k] requests of size k get their own frame (seed 100000 + j); a later
+    request of that size reuses the pooled frame of rank (its rank % per_size[k]).
+    Returns (source request index per request, {source index: image})."""
+    seen = [[] for _ in SERVED_SIZES]
+    rank = [0] * len(SERVED_SIZES)
+    src = np.empty(len(which), np.int64)
+    for j, k in enumerate(which):
+        pool = seen[k]
+        if len(pool) < per_size[k]:
+            pool.append(j)
+            src[j] = j
+        else:
+            src[j] = pool[rank[k] % per_size[k]]
+        rank[k] += 1
+    frames = {}
+    for k, pool in enumerate(seen):
+        w, h = SERVED_SIZES[k]
+        for j in pool:
+            frames[j] = generic_image(w, h, seed=100000 + j)
+    return src, frames

@@ -79,7 +79,8 @@ class SiftStages(ctypes.Structure):
 
 class SiftKernelStat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_uint64),
-                ("total_ms", ctypes.c_double), ("bytes", ctypes.c_double)]
+                ("total_ms", ctypes.c_double), ("bytes", ctypes.c_double),
+                ("ops", ctypes.c_double)]
 
 
 class SiftServerConfig(ctypes.Structure):
@@ -103,7 +104,7 @@ EXPORTED_SYMBOLS = [
     "sift_hip_server_start", "sift_hip_submit", "sift_hip_poll", "sift_hip_server_stop",
     "sift_hip_to_luma8", "sift_hip_detect_color", "sift_hip_detect_batch_color",
     "sift_hip_detect_batch_u8", "sift_result_u8_free", "sift_hip_submit_color",
-    "sift_draw_keypoints",
+    "sift_draw_keypoints", "sift_hip_tune_blur",
 ]
 
 _LIB = None
@@ -137,6 +138,11 @@ def lib():
             ctypes.c_void_p, P(SiftImage), P(SiftParams), ctypes.c_size_t, P(ctypes.c_size_t),
             P(ctypes.c_void_p), P(ctypes.c_void_p), P(ctypes.c_void_p)]
         L.sift_hip_detect_batch_device.restype = ctypes.c_int
+        L.sift_hip_tune_blur.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.c_float, ctypes.c_uint32, ctypes.c_char_p,
+                                         ctypes.c_char_p, ctypes.c_uint32, P(ctypes.c_double),
+                                         P(ctypes.c_int)]
+        L.sift_hip_tune_blur.restype = ctypes.c_int
         L.sift_result_free.argtypes = [P(SiftResult)]
         L.sift_hip_pyramid.argtypes = [ctypes.c_void_p, P(SiftImage), P(ctypes.c_uint8),
                                        P(ctypes.c_float)]
@@ -433,10 +439,11 @@ class Sift:
                "sift_hip_detect_batch_u8")
         return [_take_result_u8(outs[i]) for i in range(n)]
 
-    def detect_batch_device(self, images, params=None):
+    def detect_batch_device(self, images, params=None, ctx=None):
         """images: list of (device_ptr, w, h, stride). Returns (counts, kp_ptr,
-        desc_ptr, offsets_ptr); results stay on the device."""
-        ctx = _Ctx.get(self.device)
+        desc_ptr, offsets_ptr); results stay on the device. ctx: a context from
+        new_context() (default: the process-global one)."""
+        ctx = ctx if ctx is not None else _Ctx.get(self.device)
         n = len(images)
         ims = (SiftImage * n)()
         for i, (ptr, w, h, stride) in enumerate(images):
@@ -519,6 +526,41 @@ class Sift:
         return cand, ref, kps, desc
 
 
+# blur forms of the tuning table (sift_hip_tune_blur)
+BLUR_H_FORMS = ("h8", "h2", "hsmall", "hgeneric")
+BLUR_V_FORMS = ("v", "vnarrow", "vsmall", "vgeneric")
+
+
+def tune_blur(width, height, sigma, jobs, h_form=None, v_form=None, reps=10, device=0):
+    """(horizontal ms, vertical ms, horizontal form, vertical form) of one blur of
+    sigma on `jobs` levels of width x height with the given forms (None = the
+    built-in choice), or None when a form is not valid for the level."""
+    ctx = _Ctx.get(device)
+    out = (ctypes.c_double * 2)()
+    forms = (ctypes.c_int * 2)()
+    rc = lib().sift_hip_tune_blur(ctx, width, height, sigma, jobs,
+                                  h_form.encode() if h_form else None,
+                                  v_form.encode() if v_form else None, reps, out, forms)
+    if rc == -1 and "not valid" in last_error():
+        return None
+    _check(rc, "sift_hip_tune_blur")
+    return out[0], out[1], BLUR_H_FORMS[forms[0]], BLUR_V_FORMS[forms[1]]
+
+
+def new_context(device=0):
+    """A context of its own (sift_hip_create with the default params; it reads
+    the tuning table as it is at this call); free it with free_context."""
+    p = SiftParams()
+    lib().sift_params_default(ctypes.byref(p))
+    h = ctypes.c_void_p()
+    _check(lib().sift_hip_create(ctypes.byref(p), device, ctypes.byref(h)), "sift_hip_create")
+    return h
+
+
+def free_context(h):
+    lib().sift_hip_destroy(h)
+
+
 def profile(enable=True, device=0, only=None):
     """sift_hip_profile: per-kernel HIP-event timing of the context's launches
     (resets the counters). only: time just that kernel's launches (e.g.
@@ -529,13 +571,15 @@ def profile(enable=True, device=0, only=None):
 
 
 def profile_read(device=0):
-    """{kernel: (launches, total_ms, algorithmic_bytes)} since profile(True)."""
+    """{kernel: (launches, total_ms, algorithmic_bytes, prescribed_f32_ops)} since
+    profile(True)."""
     ctx = _Ctx.get(device)
     n = ctypes.c_size_t()
     _check(lib().sift_hip_profile_read(ctx, None, 0, ctypes.byref(n)), "sift_hip_profile_read")
     arr = (SiftKernelStat * max(n.value, 1))()
     _check(lib().sift_hip_profile_read(ctx, arr, n.value, ctypes.byref(n)), "sift_hip_profile_read")
-    return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms), float(arr[i].bytes))
+    return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms), float(arr[i].bytes),
+                                   float(arr[i].ops))
             for i in range(n.value)}
 
 
@@ -555,8 +599,9 @@ class SiftServer:
         cfg = SiftServerConfig(hbm_budget_bytes, max_batch, max_queue)
         rc = lib().sift_hip_server_start(self._ctx, ctypes.byref(cfg))
         if rc != SIFT_OK:
+            msg = last_error()
             lib().sift_hip_destroy(self._ctx)
-            raise SiftError(f"sift_hip_server_start failed ({rc})")
+            raise SiftError(f"sift_hip_server_start failed ({rc}): {msg}")
         self._keep = {}
 
     def submit(self, img, tag, sift=None, fmt=None):
@@ -573,7 +618,7 @@ class SiftServer:
             img, s = _color_struct(img, fmt)
             rc = lib().sift_hip_submit_color(self._ctx, ctypes.byref(s), pp, tag)
         if rc != SIFT_OK:
-            raise SiftError(f"sift_hip_submit failed ({rc})")
+            raise SiftError(f"sift_hip_submit failed ({rc}): {last_error()}")
 
     def poll(self, cap=256, timeout_ms=100):
         arr = (SiftCompletion * cap)()

@@ -30,8 +30,13 @@ def _fresh_detect(img, octaves):
         L.sift_hip_destroy(h)
 
 
+@pytest.mark.parametrize("queue", ["0", "1"])
 @pytest.mark.parametrize("case", ["512x512", "1920x1080"])
-def test_overflow_and_retry_gives_the_same_result(case, monkeypatch):
+def test_overflow_and_retry_gives_the_same_result(case, queue, monkeypatch):
+    """Run with the static keypoint assignment and with the device work queue
+    (SIFT_HIP_QUEUE=1), whose counters must be cleared again for the pass that
+    follows an overflow."""
+    monkeypatch.setenv("SIFT_HIP_QUEUE", queue)
     img, octv, ct = imgen.config_image(case)
     ref = sift_hip.Sift(1.6, octv, 3, 0.5, ct, 10.0).detect_and_compute(img)
     assert len(ref[0]) > 512

@@ -26,7 +26,9 @@ def _same(a, b):
     return a[0].tobytes() == b[0].tobytes() and a[1].tobytes() == b[1].tobytes()
 
 
-def test_replays_match_eager():
+@pytest.mark.parametrize("queue", ["0", "1"])
+def test_replays_match_eager(queue, monkeypatch):
+    monkeypatch.setenv("SIFT_HIP_QUEUE", queue)
     s = sift_hip.Sift(1.6, 5, 3, 0.5, 0.04, 10.0)
     a = imgen.generic_image(640, 480, seed=41)
     b = imgen.generic_image(640, 480, seed=42)
@@ -44,7 +46,9 @@ def test_replays_match_eager():
             assert np.array_equal(sift_hip.to_luma8(rgb), luma)
 
 
-def test_batch_replays_match_eager():
+@pytest.mark.parametrize("queue", ["0", "1"])
+def test_batch_replays_match_eager(queue, monkeypatch):
+    monkeypatch.setenv("SIFT_HIP_QUEUE", queue)
     shapes = [(640, 480), (97, 61), (1280, 720)]
     imgs = [imgen.generic_image(w, h, seed=200 + i) for i, (w, h) in enumerate(shapes)]
     s = sift_hip.Sift.default()

@@ -95,6 +95,41 @@ def test_candidates_bit_exact_interval_counts(S):
         assert alone[1].tobytes() == together[i][1].tobytes(), i
 
 
+def test_row_wise_extrema_seven_intervals():
+    """Seven intervals (nine DoG layers) are beyond the tiled extremum kernel's
+    instantiations (1..6) and take the row-wise k_extrema: candidates and refined
+    keypoints bit-exact against the oracle, final keypoints and descriptors
+    within tolerance."""
+    img = imgen.generic_image(640, 480, seed=57)
+    s, p = _sift(octaves=4, S=7)
+    cand, ref, kps, desc = s.detect_stages(img)
+    _, _, ocand, oref, _ = oracle.detect_and_compute(p, img, stages=True)
+    assert len(cand) > 0
+    assert cand.tobytes() == ocand.tobytes()
+    for f in ("x", "y", "response", "octave", "layer"):
+        assert np.array_equal(ref[f], oref[f]), f
+    _, _, _, m = _parity(img, s, p)
+    assert m["n_ref"] > 0
+    assert m["precision"] >= MIN_PR and m["recall"] >= MIN_PR
+    assert m["desc_l2_max"] <= DESC_L2_TOL
+
+
+def test_batch_mixing_interval_counts():
+    """A batch whose frames use different interval counts (3 and 4) runs the
+    row-wise extremum kernel for all of them; each frame equals its result alone
+    (where the tiled kernel runs), bit for bit."""
+    shapes = [(640, 480), (800, 600), (333, 221), (1280, 720)]
+    imgs = [imgen.generic_image(w, h, seed=80 + i) for i, (w, h) in enumerate(shapes)]
+    params = [sift_hip.Sift(1.6, imgen.octaves_for(w, h), 3 + (i % 2), 0.5, 0.04, 10.0)
+              for i, (w, h) in enumerate(shapes)]
+    together = sift_hip.Sift.default().detect_batch(imgs, params)
+    for i, img in enumerate(imgs):
+        alone = params[i].detect_and_compute(img)
+        assert len(alone[0]) > 0, i
+        assert alone[0].tobytes() == together[i][0].tobytes(), i
+        assert alone[1].tobytes() == together[i][1].tobytes(), i
+
+
 def _parity(img, s, p):
     kps, desc = s.detect_and_compute(img)
     okps, odesc = oracle.detect_and_compute(p, img)
@@ -228,3 +263,26 @@ def test_high_contrast_threshold():
     assert len(kd) > 0
     assert len(kh) < len(kd) // 2 or len(kh) == 0
     assert len(kh) == len(dh)
+
+
+def test_work_queue_equals_static_assignment(monkeypatch):
+    """The orientation and descriptor kernels give the same results whether their
+    work-groups take keypoints from the device work queue or by the static
+    grid-stride assignment (the default), on a single frame and a batch."""
+    imgs = [imgen.generic_image(w, h, seed=90 + i) for i, (w, h) in
+            enumerate([(1280, 720), (640, 480), (1920, 1080), (333, 221)])]
+    params = [sift_hip.Sift(1.6, imgen.octaves_for(w, h), 3, 0.5, 0.04, 10.0) for (h, w) in
+              (im.shape for im in imgs)]
+    s = sift_hip.Sift.default()
+    monkeypatch.setenv("SIFT_HIP_QUEUE", "1")
+    queued = s.detect_batch(imgs, params)
+    single = params[2].detect_and_compute(imgs[2])
+    monkeypatch.setenv("SIFT_HIP_QUEUE", "0")
+    static = s.detect_batch(imgs, params)
+    single_static = params[2].detect_and_compute(imgs[2])
+    for i in range(len(imgs)):
+        assert len(queued[i][0]) > 0, i
+        assert queued[i][0].tobytes() == static[i][0].tobytes(), i
+        assert queued[i][1].tobytes() == static[i][1].tobytes(), i
+    assert single[0].tobytes() == single_static[0].tobytes()
+    assert single[1].tobytes() == single_static[1].tobytes()
