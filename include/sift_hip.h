@@ -120,6 +120,29 @@ int sift_hip_detect_batch_device(sift_hip_ctx* ctx, const sift_image_t* images,
                                  const sift_keypoint_t** d_keypoints,
                                  const float** d_descriptors, const uint32_t** d_offsets);
 
+/* No counterpart in the reference: sift_hip_detect_batch_device for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). It calls no device or pinned
+ * allocator: its working set (pyramids, temporaries, candidate lists, weight
+ * tables, job records) lives in one static device array of
+ * SIFT_HIP_SCRATCH_BYTES per GPU, held from the library's first use on that GPU
+ * until the process ends, and it writes the results to the caller's buffers:
+ * d_keypoints / d_descriptors (device, keypoint_capacity entries, 128 f32 each),
+ * d_offsets (device, n + 1 uint32 per-image keypoint offsets) and out_counts
+ * (host, n entries, may be NULL). It synchronises once, like the variant above.
+ * Returns SIFT_ERR_NOMEM, with a message, when the batch's working set exceeds
+ * SIFT_HIP_SCRATCH_BYTES, and when the keypoints exceed keypoint_capacity (then
+ * out_counts and d_offsets hold the counts and offsets found, and nothing past
+ * keypoint_capacity is written). All calls on one GPU share its array, from any
+ * context or thread: calls that overlap on one GPU race and must be serialised
+ * by the caller. */
+#define SIFT_HIP_SCRATCH_BYTES ((size_t)256 << 20)
+int sift_hip_detect_batch_device_noalloc(sift_hip_ctx* ctx, const sift_image_t* images,
+                                         const sift_params_t* params, size_t n,
+                                         sift_keypoint_t* d_keypoints, float* d_descriptors,
+                                         size_t keypoint_capacity, uint32_t* d_offsets,
+                                         size_t* out_counts);
+
 void sift_result_free(sift_result_t* r);
 
 /* Pyramid introspection: initial blur + Gaussian pyramid + DoG pyramid

@@ -6,6 +6,7 @@
 // (src/sift.rs:132-147), initial blur (src/sift.rs:1205-1209), imageproc's
 // Gaussian taps and image's Lanczos3 weights. These feed the kernels as tables.
 #include <dlfcn.h>
+#include <limits.h>
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -310,11 +311,13 @@ void Context::release() {
     if (tables.bounce) (void)hipHostFree(tables.bounce);
     if (pinned) (void)hipHostFree(pinned);
     if (pinned_out) (void)hipHostFree(pinned_out);
+    if (ext_pinned) (void)hipHostFree(ext_pinned);
     if (stream) (void)hipStreamDestroy(stream);
     arena = nullptr;
     tables.dev = nullptr;
     tables.bounce = nullptr;
-    pinned = pinned_out = nullptr;
+    pinned = pinned_out = ext_pinned = nullptr;
+    ext_pinned_cap = 0;
     stream = nullptr;
     arena_cap = tables.dev_cap = tables.bounce_cap = pinned_cap = pinned_out_cap = 0;
 }
@@ -535,7 +538,7 @@ void Context::prof_begin(int kernel, double bytes, hipEvent_t* a) {
     (void)kernel;
     (void)bytes;
     *a = nullptr;
-    if (!profiling || (prof_only >= 0 && kernel != prof_only)) return;
+    if (!profiling || prof_blocked || (prof_only >= 0 && kernel != prof_only)) return;
     *a = prof_event();
     if (*a) (void)hipEventRecord(*a, stream);
 }
@@ -593,11 +596,13 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     }
     // the previous call synchronised the stream before returning, so the table
     // bounce buffer and the pinned staging are free to reuse
-    int rc = tables.ensure_uploaded(stream);
+    // (a call in external memory copies the whole table store to its base instead)
+    int rc = B.ext ? SIFT_OK : tables.ensure_uploaded(stream);
     if (rc) return rc;
 
     // --- device layout ---
     Carver C;
+    if (B.ext) C.off = round_up(tables.host.size(), 256);  // the tables come first
     std::vector<size_t> in_off(n), htemp_off(n), rtmp_off(n);
     std::vector<std::vector<size_t>> lev_off(n), dog_off(n);
     std::vector<int> in_pitch(n);
@@ -684,9 +689,14 @@ int Context::run(const Batch& B, RunOutputs* outs) {
 
     // capacities (grow-and-retry on overflow)
     size_t want_cap = std::max<size_t>(initial_cap_min(), pix_layers / initial_cap_divisor());
-    if (cand_cap < want_cap) cand_cap = want_cap;
-    if (kp_cap < cand_cap + cand_cap / 2) kp_cap = cand_cap + cand_cap / 2;
-    const int cap = (int)cand_cap, kcap = (int)kp_cap;
+    if (!B.ext) {
+        if (cand_cap < want_cap) cand_cap = want_cap;
+        if (kp_cap < cand_cap + cand_cap / 2) kp_cap = cand_cap + cand_cap / 2;
+    }
+    // external memory: the call's own candidate capacity, the caller's output capacity
+    const int cap = B.ext ? (int)std::min<size_t>(std::max(B.ext->cand_cap, want_cap), INT_MAX / 256)
+                          : (int)cand_cap;
+    const int kcap = B.ext ? (int)std::min<size_t>(B.ext->kcap, INT_MAX / 512) : (int)kp_cap;
 
     const size_t bits_off = C.take((size_t)std::max<long long>(nbits, 1) * 8);
     const size_t counts_off = C.take((size_t)(nslots + kCountExtra) * 4);  // + lists and queue heads
@@ -700,8 +710,8 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     const size_t ang_off = C.take((size_t)cap * kMaxPeaks * 4);
     const size_t koff_off = C.take((size_t)(cap + 2) * 4);
     const size_t scantmp2_off = C.take((size_t)(cap / kScanTileElems + 4) * 4);
-    const size_t kps_off = C.take((size_t)kcap * sizeof(Keypoint));
-    const size_t desc_off = C.take((size_t)kcap * 128 * 4);
+    const size_t kps_off = B.ext ? 0 : C.take((size_t)kcap * sizeof(Keypoint));
+    const size_t desc_off = B.ext ? 0 : C.take((size_t)kcap * 128 * 4);
     const size_t summary_off = C.take((size_t)(n + 3) * 4);
     const size_t wide_off = C.take((size_t)cap * 4);  // keypoints for the wide-window descriptor
     const size_t mid_off = C.take((size_t)cap * 4);   // keypoints for the R <= 31 descriptor
@@ -726,7 +736,11 @@ int Context::run(const Batch& B, RunOutputs* outs) {
                               octs.size() * sizeof(OctaveDesc) + (size_t)n * sizeof(ImageParams) +
                               (size_t)(n + 1) * sizeof(int) + 4096;
     const size_t blob_dev_off = round_up(C.off, 256);
-    if (arena_limit && blob_dev_off + blob_bound > arena_limit &&
+    if (B.ext && blob_dev_off + blob_bound > B.ext->cap)
+        return fail(SIFT_ERR_NOMEM, "batch needs " + std::to_string(blob_dev_off + blob_bound) +
+                                        " bytes of scratch; the no-allocation entry point has " +
+                                        std::to_string(B.ext->cap) + " per GPU");
+    if (!B.ext && arena_limit && blob_dev_off + blob_bound > arena_limit &&
         (cand_cap > want_cap || kp_cap > want_cap + want_cap / 2)) {
         // capacities grown by an earlier, larger batch do not fit this lane's
         // share: lay this batch out with its own
@@ -734,10 +748,12 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         kp_cap = want_cap + want_cap / 2;
         return run(B, outs);
     }
-    rc = ensure_arena(blob_dev_off + blob_bound);
-    if (rc) return rc;
-    char* A = arena;
-    char* T = tables.dev;
+    if (!B.ext) {
+        rc = ensure_arena(blob_dev_off + blob_bound);
+        if (rc) return rc;
+    }
+    char* A = B.ext ? B.ext->base : arena;
+    char* T = B.ext ? B.ext->base : tables.dev;
     auto AP = [&](size_t off) { return A + off; };
 
     std::vector<BlurJob> hj, hj2, hgj, hsj, vj, vnj, vgj, vsj, cj;
@@ -987,16 +1003,31 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     if (blob.size() > blob_bound) return fail(SIFT_ERR_DEVICE, "internal: job blob bound exceeded");
 
     hipStream_t st = stream;
+    if (B.ext) {
+        // the table store and the blob through the staging allocated with the
+        // context (the previous call synchronised before returning)
+        const size_t tb = round_up(tables.host.size(), 256);
+        if (tb + blob.size() + kExtSummaryBytes > ext_pinned_cap)
+            return fail(SIFT_ERR_NOMEM, "weight tables and job records (" + std::to_string(tb + blob.size()) +
+                                            " bytes) exceed the no-allocation staging");
+        memcpy(ext_pinned, tables.host.data(), tables.host.size());
+        memcpy(ext_pinned + tb, blob.data(), blob.size());
+        if (!tables.host.empty())
+            HIP_TRY(hipMemcpyAsync(B.ext->base, ext_pinned, tables.host.size(), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(A + blob_dev_off, ext_pinned + tb, blob.size(), hipMemcpyHostToDevice, st));
+    }
     // staging: inputs + blob in pinned memory
     size_t in_bytes = 0;
     if (!B.device_input)
         for (int i = 0; i < n; i++)
             in_bytes += (size_t)B.images[i].width * B.images[i].height * pixel_bytes(B.format(i));
-    rc = ensure_pinned(round_up(blob.size(), 256) + in_bytes + 256);
-    if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(stream));  // pinned staging is reused across calls
-    memcpy(pinned, blob.data(), blob.size());
-    HIP_TRY(hipMemcpyAsync(A + blob_dev_off, pinned, blob.size(), hipMemcpyHostToDevice, st));
+    if (!B.ext) {
+        rc = ensure_pinned(round_up(blob.size(), 256) + in_bytes + 256);
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(stream));  // pinned staging is reused across calls
+        memcpy(pinned, blob.data(), blob.size());
+        HIP_TRY(hipMemcpyAsync(A + blob_dev_off, pinned, blob.size(), hipMemcpyHostToDevice, st));
+    }
     if (!B.device_input) {
         size_t po = round_up(blob.size(), 256);
         for (int i = 0; i < n; i++) {
@@ -1026,8 +1057,8 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     unsigned* d_npk = (unsigned*)AP(npk_off);
     float* d_ang = (float*)AP(ang_off);
     unsigned* d_koff = (unsigned*)AP(koff_off);
-    Keypoint* d_kps = (Keypoint*)AP(kps_off);
-    float* d_desc = (float*)AP(desc_off);
+    Keypoint* d_kps = B.ext ? B.ext->kps : (Keypoint*)AP(kps_off);
+    float* d_desc = B.ext ? B.ext->desc : (float*)AP(desc_off);
     unsigned* d_sum = (unsigned*)AP(summary_off);
     unsigned* d_total = d_offs + nslots;
 
@@ -1111,7 +1142,8 @@ int Context::run(const Batch& B, RunOutputs* outs) {
                           (int*)AP(mid_off), d_counts + nslots + 2 + kNumXcd, st);
         prof_end(K_DESCRIPTOR, 0.0, ev);
         prof_begin(K_FINALIZE, 0.0, &ev);
-        launch_finalize(d_offs, (const int*)(D + isb_blob), n, d_koff, d_total, cap, d_sum, st);
+        launch_finalize(d_offs, (const int*)(D + isb_blob), n, d_koff, d_total, cap, d_sum,
+                        B.ext ? B.ext->offsets : nullptr, st);
         prof_end(K_FINALIZE, 0.0, ev);
         if (B.desc_u8) {
             prof_begin(K_DESC_U8, 0.0, &ev);
@@ -1122,7 +1154,8 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         return SIFT_OK;
     };
 
-    if (!use_graphs || profiling || !graphs_enabled_by_env()) {
+    // (a capture and its instantiation allocate: external-memory calls launch eagerly)
+    if (B.ext || !use_graphs || profiling || !graphs_enabled_by_env()) {
         rc = issue(st);
         if (rc) return rc;
     } else {
@@ -1283,6 +1316,43 @@ void load_tune_table(std::map<sh::TuneKey, std::pair<int, int>>& out) {
     }
 }
 
+}  // namespace
+
+// ===========================================================================
+// Scratch of the no-allocation entry point
+// ===========================================================================
+// sift_hip_detect_batch_device_noalloc works in this array instead of a
+// context's arena. Each GPU holds its own copy, at its own address, from the
+// library's first use on it until the process ends. In host code `scratch`
+// only names the symbol: its device address on each GPU is looked up once,
+// with that GPU current, when the first context on it is created, and only
+// that address (g_scratch) is handed to kernels and copies.
+static __device__ __align__(256) char scratch[SIFT_HIP_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch[kMaxScratchDevices] = {};
+hipError_t g_scratch_err[kMaxScratchDevices] = {};
+
+// Looks up this device's scratch address (the device is current). A failed
+// lookup does not fail context creation: only the no-allocation entry point
+// needs the address, and it reports the error.
+void scratch_setup(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p) g_scratch[device] = (char*)p;
+    else g_scratch_err[device] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+}
+
+char* scratch_of(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch[device];
+}
 }  // namespace
 
 // ===========================================================================
@@ -1453,6 +1523,13 @@ int sift_hip_create(const sift_params_t* params, int device, sift_hip_ctx** out)
     C->device = device;
     C->params = *params;
     load_tune_table(C->tune);
+    scratch_setup(device);
+    // staging of the no-allocation entry point (host memory; a failure here only
+    // disables that entry point)
+    if (hipHostMalloc((void**)&C->ext_pinned, kExtStagingBytes, hipHostMallocDefault) == hipSuccess)
+        C->ext_pinned_cap = kExtStagingBytes;
+    else
+        C->ext_pinned = nullptr;
     e = hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete C;
@@ -1638,6 +1715,70 @@ int sift_hip_detect_batch_device(sift_hip_ctx* ctx, const sift_image_t* images,
         return SIFT_OK;
     }
     return fail(SIFT_ERR_NOMEM, "keypoint capacity did not converge");
+}
+
+int sift_hip_detect_batch_device_noalloc(sift_hip_ctx* ctx, const sift_image_t* images,
+                                         const sift_params_t* params, size_t n,
+                                         sift_keypoint_t* d_keypoints, float* d_descriptors,
+                                         size_t keypoint_capacity, uint32_t* d_offsets,
+                                         size_t* out_counts) {
+    if (!ctx || !images || n == 0 || !d_offsets || (keypoint_capacity && (!d_keypoints || !d_descriptors)))
+        return fail(SIFT_ERR_INVALID, "null argument or empty batch");
+    sh::Context* C = reinterpret_cast<sh::Context*>(ctx);
+    for (size_t i = 0; i < n; i++) {
+        int rc = check_image(&images[i]);
+        if (rc) return rc;
+        if (params && (rc = check_params(&params[i]))) return rc;
+    }
+    std::lock_guard<std::mutex> lk(C->mu);
+    (void)hipSetDevice(C->device);
+    char* base = scratch_of(C->device);
+    if (!base) {
+        const hipError_t e = C->device >= 0 && C->device < kMaxScratchDevices ? g_scratch_err[C->device]
+                                                                             : hipErrorInvalidDevice;
+        return fail(SIFT_ERR_DEVICE, std::string("scratch address of this device unknown: ") + hipGetErrorString(e));
+    }
+    sh::ExternalMem X;
+    X.base = base;
+    X.cap = SIFT_HIP_SCRATCH_BYTES;
+    X.kps = reinterpret_cast<sh::Keypoint*>(d_keypoints);
+    X.desc = d_descriptors;
+    X.kcap = keypoint_capacity;
+    X.offsets = d_offsets;
+    if (!C->ext_pinned || (n + 3) * 4 > kExtSummaryBytes)
+        return fail(SIFT_ERR_NOMEM, !C->ext_pinned ? "no staging for the no-allocation entry point"
+                                                   : "batch too large for the no-allocation entry point");
+    sh::Batch B;
+    B.ext = &X;
+    B.images.assign(images, images + n);
+    B.params.assign(n, C->params);
+    if (params) B.params.assign(params, params + n);
+    B.device_input = true;
+    C->prof_blocked = true;  // event timing would create events
+    struct Unblock {
+        sh::Context* c;
+        ~Unblock() { c->prof_blocked = false; }
+    } unblock{C};
+    // the summary comes back through the end of the staging
+    unsigned* sum = reinterpret_cast<unsigned*>(C->ext_pinned + C->ext_pinned_cap - kExtSummaryBytes);
+    for (int attempt = 0; attempt < 4; attempt++) {
+        sh::RunOutputs R;
+        int rc = C->run(B, &R);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(sum, R.d_summary, (n + 3) * 4, hipMemcpyDeviceToHost, C->stream));
+        HIP_TRY(hipStreamSynchronize(C->stream));
+        if (sum[0] > (unsigned)R.cap) {  // candidates overflowed: grow within the scratch
+            X.cand_cap = (size_t)sum[0] + sum[0] / 4 + 1024;
+            continue;
+        }
+        if (out_counts)
+            for (size_t i = 0; i < n; i++) out_counts[i] = sum[3 + i] - sum[2 + i];
+        if (sum[1] > (unsigned)R.kcap)
+            return fail(SIFT_ERR_NOMEM, "keypoint_capacity " + std::to_string(keypoint_capacity) + " < " +
+                                            std::to_string(sum[1]) + " keypoints found");
+        return SIFT_OK;
+    }
+    return fail(SIFT_ERR_NOMEM, "candidate capacity did not converge");
 }
 
 void sift_result_free(sift_result_t* r) {

@@ -1509,17 +1509,19 @@ __global__ __launch_bounds__(kThreads) void k_descriptor_wide(
     }
 }
 
-// Per-image keypoint ranges: summary[2+i] = koff[first candidate of image i];
-// summary[0] = total candidates (may exceed cap), summary[1] = total keypoints.
+// Per-image keypoint ranges: summary[2+i] = koff[first candidate of image i]
+// (and out_offsets[i], when given); summary[0] = total candidates (may exceed
+// cap), summary[1] = total keypoints.
 __global__ void k_finalize(const unsigned* __restrict__ slot_offsets,
                            const int* __restrict__ img_slot_begin, int nimg,
                            const unsigned* __restrict__ koff, const unsigned* __restrict__ total,
-                           int cap, unsigned* __restrict__ summary) {
+                           int cap, unsigned* __restrict__ summary, unsigned* __restrict__ out_offsets) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > nimg) return;
     const unsigned n = min(*total, (unsigned)cap);
     const unsigned cb = min(slot_offsets[img_slot_begin[i]], n);
     summary[2 + i] = koff[cb];
+    if (out_offsets) out_offsets[i] = koff[cb];
     if (i == 0) {
         summary[0] = *total;
         summary[1] = koff[n];
@@ -1681,9 +1683,9 @@ void launch_descriptor(const OctaveDesc* d_oct, const ImageParams* d_params, con
 
 void launch_finalize(const unsigned* d_slot_offsets, const int* d_img_slot_begin, int nimg,
                      const unsigned* d_koff, const unsigned* d_total, int cap, unsigned* d_summary,
-                     hipStream_t s) {
+                     unsigned* d_out_offsets, hipStream_t s) {
     hipLaunchKernelGGL(k_finalize, dim3((nimg + 1 + 255) / 256), dim3(256), 0, s, d_slot_offsets,
-                       d_img_slot_begin, nimg, d_koff, d_total, cap, d_summary);
+                       d_img_slot_begin, nimg, d_koff, d_total, cap, d_summary, d_out_offsets);
 }
 
 void launch_desc_u8(const float* d_desc, const unsigned* d_count, int kcap, uint8_t* d_out,

@@ -71,7 +71,28 @@ struct TableStore {
     int ensure_uploaded(hipStream_t stream);
 };
 
+// Pinned staging of the no-allocation entry point (per context, allocated at
+// creation) and the part of it that receives the summary (up to 16K frames).
+constexpr size_t kExtStagingBytes = (size_t)4 << 20;
+constexpr size_t kExtSummaryBytes = (size_t)64 << 10;
+
+// Device memory a call works in instead of the context's own (the
+// no-allocation entry point): the weight tables at base, then the call's
+// layout, all within [base, base + cap); keypoints and descriptors go to the
+// caller's buffers (kcap entries). cand_cap: candidate capacity of the call (0 =
+// the initial rule).
+struct ExternalMem {
+    char* base = nullptr;
+    size_t cap = 0;
+    Keypoint* kps = nullptr;
+    float* desc = nullptr;
+    size_t kcap = 0;
+    size_t cand_cap = 0;
+    unsigned* offsets = nullptr;  // the caller's n + 1 per-image keypoint offsets
+};
+
 struct Batch {
+    const ExternalMem* ext = nullptr;  // set: no allocation, no graphs, no timing
     std::vector<sift_image_t> images;
     std::vector<uint32_t> formats;  // SIFT_PIXEL_* per image; empty = all grey
     std::vector<sift_params_t> params;
@@ -171,8 +192,13 @@ struct Context {
     uint64_t graph_clock = 0;
     void clear_graphs();
 
+    // Pinned staging of the external-memory calls (tables and job blob in, the
+    // summary out), allocated with the context so that those calls allocate nothing.
+    char* ext_pinned = nullptr;
+    size_t ext_pinned_cap = 0;
 
     bool profiling = false;
+    bool prof_blocked = false;  // no event timing (it creates events) during a no-allocation call
     int prof_only = -1;  // kernel id whose launches are timed, -1 = all
     std::vector<ProfPending> prof_pending;
     std::vector<hipEvent_t> prof_events;  // free pool

@@ -166,10 +166,11 @@ void launch_descriptor(const OctaveDesc* d_oct, const ImageParams* d_params, con
                        int* d_wide_list, unsigned* d_mid_count, int* d_mid_list, unsigned* d_queues,
                        hipStream_t s);
 // Writes summary[0] = candidates, summary[1] = keypoints, summary[2..2+nimg] =
-// per-image keypoint offsets (nimg+1 entries).
+// per-image keypoint offsets (nimg+1 entries), the offsets also to
+// d_out_offsets when it is not null.
 void launch_finalize(const unsigned* d_slot_offsets, const int* d_img_slot_begin, int nimg,
                      const unsigned* d_koff, const unsigned* d_total, int cap,
-                     unsigned* d_summary, hipStream_t s);
+                     unsigned* d_summary, unsigned* d_out_offsets, hipStream_t s);
 // u8 descriptors of the WebGPU backends: out = trunc(clamp(d * 512, 0, 255)) for
 // the first min(*d_count, kcap) descriptors.
 void launch_desc_u8(const float* d_desc, const unsigned* d_count, int kcap, uint8_t* d_out,

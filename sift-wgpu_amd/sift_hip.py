@@ -104,8 +104,11 @@ EXPORTED_SYMBOLS = [
     "sift_hip_server_start", "sift_hip_submit", "sift_hip_poll", "sift_hip_server_stop",
     "sift_hip_to_luma8", "sift_hip_detect_color", "sift_hip_detect_batch_color",
     "sift_hip_detect_batch_u8", "sift_result_u8_free", "sift_hip_submit_color",
-    "sift_draw_keypoints", "sift_hip_tune_blur",
+    "sift_draw_keypoints", "sift_hip_detect_batch_device_noalloc", "sift_hip_tune_blur",
 ]
+
+# Bytes of the per-GPU scratch of sift_hip_detect_batch_device_noalloc.
+SCRATCH_BYTES = 256 << 20
 
 _LIB = None
 _LIB_LOCK = threading.Lock()
@@ -138,6 +141,10 @@ def lib():
             ctypes.c_void_p, P(SiftImage), P(SiftParams), ctypes.c_size_t, P(ctypes.c_size_t),
             P(ctypes.c_void_p), P(ctypes.c_void_p), P(ctypes.c_void_p)]
         L.sift_hip_detect_batch_device.restype = ctypes.c_int
+        L.sift_hip_detect_batch_device_noalloc.argtypes = [
+            ctypes.c_void_p, P(SiftImage), P(SiftParams), ctypes.c_size_t, ctypes.c_void_p,
+            ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, P(ctypes.c_size_t)]
+        L.sift_hip_detect_batch_device_noalloc.restype = ctypes.c_int
         L.sift_hip_tune_blur.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                          ctypes.c_float, ctypes.c_uint32, ctypes.c_char_p,
                                          ctypes.c_char_p, ctypes.c_uint32, P(ctypes.c_double),
@@ -460,6 +467,24 @@ class Sift:
                                                   ctypes.byref(de), ctypes.byref(of)),
                "sift_hip_detect_batch_device")
         return list(counts), kp.value, de.value, of.value
+
+    def detect_batch_device_noalloc(self, images, d_keypoints, d_descriptors, capacity,
+                                    d_offsets, params=None):
+        """sift_hip_detect_batch_device_noalloc: images as for detect_batch_device;
+        results into the caller's device buffers (capacity keypoints of 28 B and
+        descriptors of 512 B, n + 1 uint32 offsets). Returns (rc, counts, message):
+        rc SIFT_OK, or an error code with counts the keypoints found (when the
+        capacity was too small) and the library's message."""
+        ctx = _Ctx.get(self.device)
+        n = len(images)
+        ims = (SiftImage * n)()
+        for i, (ptr, w, h, stride) in enumerate(images):
+            ims[i] = SiftImage(ptr, w, h, stride)
+        pa = self._params_array(n, params)
+        counts = (ctypes.c_size_t * n)()
+        rc = lib().sift_hip_detect_batch_device_noalloc(ctx, ims, pa, n, d_keypoints, d_descriptors,
+                                                         capacity, d_offsets, counts)
+        return rc, list(counts), (last_error() if rc != SIFT_OK else "")
 
     def pyramid(self, img):
         """Gaussian (u8) and DoG (f32) pyramids: lists over octaves of arrays
