@@ -24,7 +24,7 @@ namespace {
 
 constexpr float kPi = 3.14159274101257324219f;  // std::f32::consts::PI
 constexpr float kTwoPi = 2.0f * kPi;
-constexpr float kDescFix = 1048576.0f;  // 2^20: u32 fixed point of the descriptor sums
+constexpr float kDescFix = 1048576.0f;  // 2^20: u32 fixed point of the descriptor sums, times a per-keypoint power-of-two gain
 constexpr int kThreads = 256;
 constexpr int kExtChunk = 1024;  // pixels of one row per chunk in k_extrema
 constexpr int kScanTile = 4096;  // 256 threads x 16 elements
@@ -965,9 +965,63 @@ __device__ __forceinline__ float wave_sum_dpp(float v) {
     return __int_as_float(__builtin_amdgcn_readlane(x, 63));
 }
 
+// Maximum over the 64 lanes of a fully active wave (DPP, as wave_incl_scan),
+// returned to every lane.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned x) {
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true));  // row_shr:1
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true));  // row_shr:2
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true));  // row_shr:4
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true));  // row_shr:8
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false)); // row_bcast:15
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false)); // row_bcast:31
+    return (unsigned)__builtin_amdgcn_readlane((int)x, 63);
+}
+
+// Maximum over the 64 lanes of a fully active wave, returned to every lane.
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+// Per-keypoint gain of the descriptor's fixed-point scale: the largest power of
+// two 2^e (0 <= e <= 40) with M * 2^e <= lim, where M bounds |gx| and |gy| of
+// every sample of the window and lim is their largest possible value (255 in
+// texel units, 1 in [0, 1] units). Scaled by it, a sample's contribution
+// mag * weight * fix is still at most sqrt(2) * fix, so the u32 sum bound holds,
+// while a low-contrast window's sums use the high bits of the words instead of
+// a few units of 2^-20 (whose rounding, relative to the window's small norm,
+// reached 7e-4 of L2 at 3840x2160, contrast 0.03).
+// desc_fix_gain returns 2^e, desc_fix_exp e.
+__device__ __forceinline__ int desc_fix_exp(float M, float lim) {
+    if (!(M > 0.0f)) return 0;
+    int x;
+    (void)frexpf(M, &x);                  // M in [2^(x-1), 2^x)
+    int e = (lim > 1.0f ? 8 : 0) - x;     // M * 2^e in [128, 256), or [0.5, 1)
+    if (ldexpf(M, e) > lim) e--;
+    return min(max(e, 0), 40);
+}
+__device__ __forceinline__ float desc_fix_gain(float M, float lim) {
+    return ldexpf(1.0f, desc_fix_exp(M, lim));
+}
+
+// k_descriptor's fixed-point gain exponent for one keypoint: that of its
+// patch's texel range (max - min over the waves' (max << 8) | (255 - min) words),
+// the same value in every lane.
+__device__ __forceinline__ int desc_range_exp(const unsigned* wrange) {
+    unsigned hi = 0u, lo = 0u;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; w++) {
+        hi = max(hi, wrange[w] >> 8);
+        lo = max(lo, wrange[w] & 255u);
+    }
+    const int range = (int)hi - (int)(255u - lo);
+    return __builtin_amdgcn_readfirstlane(desc_fix_exp((float)range, 255.0f));
+}
+
 constexpr int kDescMaxR = 31;    // LDS path for sigma_octave <= 3.65
 // The instantiation for windows up to R = 28 (99.5% of the keypoints of the
-// default schedule, 22.6 KB of LDS) fits 7 work-groups per CU, the R = 31 one 6.
+// default schedule, 20.1 KB of LDS, 64 VGPRs) fits 8 work-groups per CU, the R = 31 one 6.
 constexpr int kDescSmallR = 28;
 // LDS sizes of the descriptor instantiation for windows up to MR.
 template <int MR>
@@ -1117,7 +1171,10 @@ __device__ __forceinline__ void desc_bounds(const DescGeom& G, int& bx0, int& bx
 
 // Wave 0: sums the histogram copies of one peak, normalises, clips at 0.2,
 // renormalises (src/sift.rs:889-913) and writes the descriptor and keypoint.
-__device__ __forceinline__ void desc_finish(const unsigned* hist, float fix, int l,
+// The sums are divided by the fixed-point scale `fix`, or (kDivide false)
+// multiplied by its exact reciprocal inv_fix when fix is a power of two.
+template <bool kDivide>
+__device__ __forceinline__ void desc_finish(const unsigned* hist, float fix, double inv_fix, int l,
                                             const Keypoint& base, float angle, unsigned kidx,
                                             int kcap, Keypoint* __restrict__ kps_out,
                                             float* __restrict__ desc_out) {
@@ -1135,7 +1192,7 @@ __device__ __forceinline__ void desc_finish(const unsigned* hist, float fix, int
             sum += t[kOddBase + a];     // odd-pair table
             if (a == 0) sum += t[kOddBase + 8];
         }
-        v[e] = (float)((double)sum / (double)fix);
+        v[e] = kDivide ? (float)((double)sum / (double)fix) : (float)((double)sum * inv_fix);
     }
     float v0 = v[0], v1 = v[1];
     const float nrm = sqrtf(wave_sum_dpp(v0 * v0 + v1 * v1));
@@ -1184,8 +1241,9 @@ __device__ __forceinline__ void desc_finish(const unsigned* hist, float fix, int
 // test xb, yb in (-1, 4), then computes mag * exp(-(px^2+py^2)/(2 (6 sigma_o)^2))
 // and atan2 and scatters trilinearly into its copy of the histogram (even and
 // odd lanes use separate copies, which halves same-address collisions of the
-// LDS adds; integer adds: independent of the order). Wave 0 sums the copies
-// and normalises.
+// LDS adds; integer adds: independent of the order). The fixed-point scale is
+// 2^20 times the largest power of two that keeps the patch's texel range times
+// it within 255 (desc_range_exp). Wave 0 sums the copies and normalises.
 // ---------------------------------------------------------------------------
 template <int MR, bool kFromList>
 __global__ __launch_bounds__(kThreads)
@@ -1210,6 +1268,8 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 8 : 6))) void k_descripto
     __shared__ int rspan[64];  // per row: first and last column of its samples (16 bits each)
     __shared__ int rpre[65];
     __shared__ int seg_len;
+    // per wave: (max << 8) | (255 - min) of the patch texels it staged
+    __shared__ unsigned wrange[kThreads / 64];
     __shared__ float2 terms_own[kTermsInPatch ? 1 : 2 * kGS];
     __shared__ float pang[kMaxPeaks], pcos[kMaxPeaks], psin[kMaxPeaks];
     uint8_t* const patch = reinterpret_cast<uint8_t*>(pu);
@@ -1249,6 +1309,7 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 8 : 6))) void k_descripto
             const int pside = 2 * R + 6;
             const int px0 = f2i_sat(floorf(xo)) - R - 2, py0 = f2i_sat(floorf(yo)) - R - 2;
             const int px0a = px0 & ~3, poff = px0 - px0a;
+            unsigned tmax = 0u, tmin = 255u;
             if (px0 >= 0 && py0 >= 0 && px0 + pside <= G.w && py0 + pside <= G.h) {
                 // interior: aligned words [px0a, round_up(px0 + pside, 4)) of each row,
                 // which end within the row's pitch
@@ -1267,6 +1328,17 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 8 : 6))) void k_descripto
                         wv[k] = *reinterpret_cast<const unsigned*>(rowp + (size_t)v * G.pitch + 4 * wd);
                     }
                 }
+                // texel range of the staged words (a few texels left and right of
+                // the patch included: a wider range is still a bound)
+#pragma unroll
+                for (int k = 0; k < 5; k++) {
+                    if (tid + k * kThreads < tot) {
+                        const unsigned b0 = wv[k] & 255u, b1 = (wv[k] >> 8) & 255u;
+                        const unsigned b2 = (wv[k] >> 16) & 255u, b3 = wv[k] >> 24;
+                        tmax = max(max(tmax, max(b0, b1)), max(b2, b3));
+                        tmin = min(min(tmin, min(b0, b1)), min(b2, b3));
+                    }
+                }
 #pragma unroll
                 for (int k = 0; k < 5; k++) {
                     const int i = tid + k * kThreads;
@@ -1280,9 +1352,18 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 8 : 6))) void k_descripto
                 for (int i = tid; i < pside * pside; i += kThreads) {
                     const int v = (int)(((float)i + 0.5f) * inv_ps), u = i - v * pside;
                     const int xx = min(max(px0 + u, 0), G.w - 1), yy = min(max(py0 + v, 0), G.h - 1);
-                    patch[v * kPS + poff + u] = G.g[(size_t)yy * G.pitch + xx];
+                    const unsigned t = G.g[(size_t)yy * G.pitch + xx];
+                    patch[v * kPS + poff + u] = (uint8_t)t;
+                    tmax = max(tmax, t);
+                    tmin = min(tmin, t);
                 }
             }
+            // Every grid value is a convex combination of patch texels, so every
+            // sample's |gx| and |gy| is at most the patch's texel range: it sets
+            // the keypoint's fixed-point gain (read after the barrier below).
+            tmax = wave_max_u32(tmax);
+            tmin = 255u - wave_max_u32(255u - tmin);
+            if (lane == 0) wrange[wave] = (tmax << 8) | (255u - tmin);
             // the grid's column and row terms: x = x_o + i (and y = y_o + j) are the
             // same f32 sums for every point of a column (row), so their texel offset
             // and fraction are formed once
@@ -1360,13 +1441,15 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 8 : 6))) void k_descripto
                     segs = wave_incl_scan((cnt + per - 1) / per);
                 }
                 rpre[tid + 1] = segs;  // segment prefix over the rows
+                // the fixed-point scale 2^(20 + fe), passed with the segment length
+                const int fe = desc_range_exp(wrange);
                 if (tid == 0) {
                     rpre[0] = 0;
-                    seg_len = per;
+                    seg_len = per | (fe << 16);
                 }
             }
             __syncthreads();
-            const int per = seg_len;
+            const int per = seg_len & 0xffff, fe = seg_len >> 16;
             const int sg = lane * 4 + wave;  // lanes of a wave take segments far apart
             if (sg < rpre[nrows]) {
                 unsigned* hw = hist + (lane & 1) * kHistWords;  // neighbouring lanes: different copies
@@ -1384,7 +1467,7 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 8 : 6))) void k_descripto
                 const int cy = by0 + r0;
                 const float py = (float)(cy - R);
                 const float rz = sa * py * inv_bw, rw = ca * py * inv_bw;
-                const float wyf = exs[cy] * (kDescFix / 255.0f);
+                const float wyf = exs[cy] * ldexpf(kDescFix / 255.0f, fe);
                 // grid point (cx, cy) of sample (cx, cy) -- the up-left neighbour -- so that
                 // all four gradient neighbours sit at non-negative offsets
                 const float* grow = grid + (by0 + r0) * kGS;
@@ -1434,7 +1517,9 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 8 : 6))) void k_descripto
                 // empty asm so the compiler does not reuse the first reads)
                 int cr = c;
                 asm volatile("" : "+v"(cr));
-                desc_finish(hist, kDescFix, tid, ref[cr], angle, koff[cr] + (unsigned)pk, kcap, kps_out,
+                desc_finish<false>(hist, 0.0f, ldexp(1.0 / (double)kDescFix, -(seg_len >> 16)), tid,
+                                   ref[cr], angle,
+                                   koff[cr] + (unsigned)pk, kcap, kps_out,
                             desc_out);
             }
             __syncthreads();
@@ -1447,7 +1532,8 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 8 : 6))) void k_descripto
 // (R > 31, sigma_octave > 3.65; none with the default sigma and 3 intervals).
 // Direct per-sample path with global reads and the reference's f32 arithmetic;
 // 256 contiguous chunks of the (2R+1)^2 window as in k_descriptor. The fixed-point
-// scale drops below 2^20 when a copy would see more than kMaxCopySamples samples.
+// scale drops below 2^20 when a copy would see more than kMaxCopySamples samples,
+// and is raised by the gain of the largest |gx|, |gy| of the window's samples.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void k_descriptor_wide(
     const OctaveDesc* __restrict__ oct, const ImageParams* __restrict__ params,
@@ -1458,6 +1544,7 @@ __global__ __launch_bounds__(kThreads) void k_descriptor_wide(
     const unsigned* __restrict__ wide_count, const int* __restrict__ wide_list) {
     __shared__ float q[256];
     __shared__ __align__(16) unsigned hist[kHistCopies * kHistWords];
+    __shared__ float wave_gmax[kThreads / 64];
     fill_q(q);
     __syncthreads();
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -1470,7 +1557,26 @@ __global__ __launch_bounds__(kThreads) void k_descriptor_wide(
         const int R = G.R, side = G.side, nsamp = side * side;
         const float wm1 = (float)(G.w - 1), hm1 = (float)(G.h - 1);
         const int per = (nsamp + kThreads - 1) / kThreads;
-        const float fix = kDescFix * fminf(1.0f, (float)kMaxCopySamples / (float)(128 * per));
+        const int i0 = (lane * 4 + wave) * per;
+        const int iend = min(i0 + per, nsamp);
+        {
+            // the largest |gx|, |gy| of the in-image samples sets the keypoint's
+            // fixed-point gain (written per wave, read after the first barrier of
+            // the peak loop)
+            float gm = 0.0f;
+            for (int i = i0; i < iend && G.np > 0; i++) {
+                const int cy = i / side, cx = i - cy * side;
+                const float xs = G.xo + (float)(cx - R), ys = G.yo + (float)(cy - R);
+                if (xs < 0.0f || xs >= wm1 || ys < 0.0f || ys >= hm1) continue;
+                const float gx = bilinear(G.g, G.pitch, G.w, G.h, xs + 1.0f, ys, q) -
+                                 bilinear(G.g, G.pitch, G.w, G.h, xs - 1.0f, ys, q);
+                const float gy = bilinear(G.g, G.pitch, G.w, G.h, xs, ys + 1.0f, q) -
+                                 bilinear(G.g, G.pitch, G.w, G.h, xs, ys - 1.0f, q);
+                gm = fmaxf(gm, fmaxf(fabsf(gx), fabsf(gy)));
+            }
+            gm = wave_max(gm);
+            if (lane == 0) wave_gmax[wave] = gm;
+        }
         for (int pk = 0; pk < G.np; pk++) {
             const float angle = angles[(size_t)c * kMaxPeaks + pk];
             for (int i = tid; i < kHistCopies * kHistWords / 4; i += kThreads)
@@ -1478,8 +1584,9 @@ __global__ __launch_bounds__(kThreads) void k_descriptor_wide(
             __syncthreads();
             const float ca = cosf(angle), sa = sinf(angle);
             unsigned* hw = hist + (lane & 1) * kHistWords;
-            const int i0 = (lane * 4 + wave) * per;
-            const int iend = min(i0 + per, nsamp);
+            const float gmax = fmaxf(fmaxf(wave_gmax[0], wave_gmax[1]), fmaxf(wave_gmax[2], wave_gmax[3]));
+            const float fix = kDescFix * fminf(1.0f, (float)kMaxCopySamples / (float)(128 * per)) *
+                              desc_fix_gain(gmax, 1.0f);
             for (int i = i0; i < iend; i++) {
                 const int cy = i / side, cx = i - cy * side;
                 const float py = (float)(cy - R), px = (float)(cx - R);
@@ -1502,7 +1609,7 @@ __global__ __launch_bounds__(kThreads) void k_descriptor_wide(
             }
             __syncthreads();
             if (tid < 64)
-                desc_finish(hist, fix, tid, G.base, angle, G.k0 + (unsigned)pk, kcap, kps_out,
+                desc_finish<true>(hist, fix, 0.0, tid, G.base, angle, G.k0 + (unsigned)pk, kcap, kps_out,
                             desc_out);
             __syncthreads();
         }

@@ -15,7 +15,7 @@ import devmem
 import imgen
 import oracle
 import sift_hip
-from parity import DESC_L2_TOL, MIN_PR, match_keypoints
+from parity import DESC_L2_MARGIN, DESC_L2_TOL, MIN_PR, match_keypoints
 
 pytestmark = pytest.mark.gpu
 
@@ -69,6 +69,7 @@ def test_bench_step_64_frames_1080p():
           f"R {m['recall']:.5f} desc_l2_max {m['desc_l2_max']:.2e}")
     assert m["precision"] >= MIN_PR and m["recall"] >= MIN_PR
     assert m["desc_l2_max"] <= DESC_L2_TOL
+    assert m["desc_l2_max"] <= DESC_L2_MARGIN
 
 
 def test_uneven_batch_config():
@@ -95,6 +96,7 @@ def test_uneven_batch_config():
               f"P {m['precision']:.5f} R {m['recall']:.5f} desc_l2_max {m['desc_l2_max']:.2e}")
         assert m["precision"] >= MIN_PR and m["recall"] >= MIN_PR
         assert m["desc_l2_max"] <= DESC_L2_TOL
+        assert m["desc_l2_max"] <= DESC_L2_MARGIN
 
 
 def test_odd_strides_and_mixed_sizes():

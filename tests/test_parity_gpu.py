@@ -13,7 +13,7 @@ import pytest
 import imgen
 import oracle
 import sift_hip
-from parity import DESC_L2_TOL, MIN_PR, check_keypoint_properties, match_keypoints
+from parity import DESC_L2_MARGIN, DESC_L2_TOL, MIN_PR, check_keypoint_properties, match_keypoints
 
 pytestmark = pytest.mark.gpu
 
@@ -112,6 +112,7 @@ def test_row_wise_extrema_seven_intervals():
     assert m["n_ref"] > 0
     assert m["precision"] >= MIN_PR and m["recall"] >= MIN_PR
     assert m["desc_l2_max"] <= DESC_L2_TOL
+    assert m["desc_l2_max"] <= DESC_L2_MARGIN
 
 
 def test_batch_mixing_interval_counts():
@@ -146,6 +147,7 @@ def test_keypoints_and_descriptors_match_oracle(case):
           f"P {m['precision']:.5f} R {m['recall']:.5f} desc_l2_max {m['desc_l2_max']:.2e}")
     assert m["precision"] >= MIN_PR and m["recall"] >= MIN_PR
     assert m["desc_l2_max"] <= DESC_L2_TOL
+    assert m["desc_l2_max"] <= DESC_L2_MARGIN
     check_keypoint_properties(kps, img.shape[1], img.shape[0])
 
 
@@ -164,6 +166,7 @@ def test_parameter_variants(variant):
     _, _, _, m = _parity(img, s, p)
     assert m["precision"] >= MIN_PR and m["recall"] >= MIN_PR
     assert m["desc_l2_max"] <= DESC_L2_TOL
+    assert m["desc_l2_max"] <= DESC_L2_MARGIN
 
 
 @pytest.mark.parametrize("sigma,S", [(1.9, 3), (1.6, 2)])
@@ -191,6 +194,7 @@ def test_wide_windows_match_oracle():
     assert len(okps) > 0
     assert m["precision"] >= MIN_PR and m["recall"] >= MIN_PR
     assert m["desc_l2_max"] <= DESC_L2_TOL
+    assert m["desc_l2_max"] <= DESC_L2_MARGIN
 
 
 def test_uniform_image_has_no_keypoints():
