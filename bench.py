@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--capacity-seconds", type=float, default=5.0,
                     help="length of the closed-loop served-capacity leg (0 disables it)")
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="requests/s of the stream")
+    ap.add_argument("--no-tuning", action="store_true",
+                    help="skip the tuning-table-vs-built-in leg (for profiler runs)")
     ap.add_argument("--traffic", default=os.path.join(REPO, "profiles", "traffic_1080p_b64.txt"),
                     help="rocprofv3 FETCH/WRITE summary giving HBM bytes per launch")
     ap.add_argument("--configs", action="store_true",
@@ -525,7 +527,8 @@ def main():
             if dom in tr:
                 out["roofline"]["traffic"] = round(tr[dom][0] + tr[dom][1])
                 out["roofline"]["traffic_read_write"] = [round(tr[dom][0]), round(tr[dom][1])]
-    out["tuning"] = tuning_leg(sift, batch, args.steps)
+    if not args.no_tuning:
+        out["tuning"] = tuning_leg(sift, batch, args.steps)
     if args.mixed_batch > 0:
         out["mixed_batch"] = mixed_batch(args.mixed_batch, step)
     if args.serve_seconds > 0:
