@@ -367,20 +367,21 @@ struct LaunchRec {
 enum KernelId {
     K_BLUR_H = 0, K_BLUR_V, K_COPY, K_BLUR_H_GENERIC, K_RESIZE_V, K_RESIZE_H, K_EXTREMA, K_SCAN,
     K_COMPACT, K_REFINE, K_ORIENT, K_DESCRIPTOR, K_FINALIZE, K_BLUR_V_NARROW, K_BLUR_V_GENERIC,
-    K_BLUR_H2, K_BLUR_H_SMALL, K_BLUR_V_SMALL, K_LUMA, K_DESC_U8, K_RESIZE_FUSED, K_COUNT
+    K_BLUR_H2, K_BLUR_H_SMALL, K_BLUR_V_SMALL, K_LUMA, K_DESC_U8, K_RESIZE_FUSED, K_BLUR_V_SMALL8,
+    K_COUNT
 };
 // Names as rocprofv3 reports the kernels (template arguments included).
 const char* const kKernelNames[K_COUNT] = {
     "k_blur_h<8>", "k_blur_v", "k_copy_level", "k_blur_h_generic", "k_resize_v", "k_resize_h",
     "k_extrema_tile", "k_scan", "k_compact", "k_refine", "k_orient", "k_descriptor",
     "k_finalize", "k_blur_v_narrow", "k_blur_v_generic", "k_blur_h<2>", "k_blur_h_small",
-    "k_blur_v_small", "k_luma", "k_desc_u8", "k_resize_fused"};
+    "k_blur_v_small", "k_luma", "k_desc_u8", "k_resize_fused", "k_blur_v_small8"};
 
 
 double blur_job_bytes(const BlurJob& j, int kind) {
     double a = (double)j.w * j.h;
     if ((kind == K_BLUR_V || kind == K_BLUR_V_NARROW || kind == K_BLUR_V_GENERIC ||
-         kind == K_BLUR_V_SMALL) && j.dog)
+         kind == K_BLUR_V_SMALL || kind == K_BLUR_V_SMALL8) && j.dog)
         return a * (2.0 + 1.0 + 4.0);
     if (kind == K_COPY && j.dog) return a * (2.0 + 4.0);
     return a * 2.0;
@@ -418,7 +419,7 @@ double resize_job_bytes(const ResizeJob& j, int kind) {
 // per lane (k_blur_v_small); one row per work-group (k_blur_v_generic). Every
 // form computes the same exact sums, so any valid choice gives the same level.
 const char* const kBlurHNames[BH_COUNT] = {"h8", "h2", "hsmall", "hgeneric"};
-const char* const kBlurVNames[BV_COUNT] = {"v", "vnarrow", "vsmall", "vgeneric"};
+const char* const kBlurVNames[BV_COUNT] = {"v", "vnarrow", "vsmall", "vgeneric", "vsmall8"};
 
 int blur_form_id(const char* name, bool vertical) {
     if (!name) return -1;
@@ -455,6 +456,7 @@ bool blur_v_valid(int form, int w, int h, int ntaps) {
         case BV_WIDE: return true;
         case BV_NARROW: return std::min(h, 16 + 2 * r) * 64 <= 65536;
         case BV_SMALL: return h <= kSmallMaxRows && n4 <= kSmallMaxTaps4;
+        case BV_SMALL8: return n4 <= kSmallMaxTaps4 && std::min(h, 4 + 2 * r) <= kSmallMaxRows8;
         case BV_GENERIC: return true;
         default: return false;
     }
@@ -492,7 +494,8 @@ void blur_v_tiles(BlurJob& J, int form) {
     switch (form) {
         case BV_WIDE: J.tiles_x = (w + 255) / 256; J.tiles = J.tiles_x * ((h + 15) / 16); break;
         case BV_NARROW: J.tiles_x = (w + 63) / 64; J.tiles = J.tiles_x * ((h + 15) / 16); break;
-        case BV_SMALL: J.tiles_x = (w + 63) / 64; J.tiles = J.tiles_x * ((h + 3) / 4); break;
+        case BV_SMALL:
+        case BV_SMALL8: J.tiles_x = (w + 63) / 64; J.tiles = J.tiles_x * ((h + 3) / 4); break;
         default: J.tiles_x = (w + 255) / 256; J.tiles = J.tiles_x * h; break;
     }
 }
@@ -756,9 +759,9 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     char* T = B.ext ? B.ext->base : tables.dev;
     auto AP = [&](size_t off) { return A + off; };
 
-    std::vector<BlurJob> hj, hj2, hgj, hsj, vj, vnj, vgj, vsj, cj;
+    std::vector<BlurJob> hj, hj2, hgj, hsj, vj, vnj, vgj, vsj, vs8j, cj;
     const std::pair<int, int> forced = blur_forced();
-    int vn_rows = 0, hs_taps4 = 0, vs_taps4 = 0, vs_rows = 0;
+    int vn_rows = 0, hs_taps4 = 0, vs_taps4 = 0, vs_rows = 0, vs8_taps4 = 0, vs8_rows = 0;
 
     auto job_bytes = [](const auto& j, int kind) -> double {
         using J = std::decay_t<decltype(j)>;
@@ -847,6 +850,11 @@ int Context::run(const Batch& B, RunOutputs* outs) {
                 vs_taps4 = std::max(vs_taps4, n4);
                 vs_rows = std::max(vs_rows, std::min(h, 4 + 2 * r));
                 break;
+            case BV_SMALL8:
+                vs8j.push_back(V);
+                vs8_taps4 = std::max(vs8_taps4, n4);
+                vs8_rows = std::max(vs8_rows, std::min(h, 4 + 2 * r));
+                break;
             default: vgj.push_back(V); break;
         }
     };
@@ -863,10 +871,12 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         push_launch(K_BLUR_V_NARROW, vnj, 0, vn_rows);
         push_launch(K_BLUR_V_GENERIC, vgj, 0, 0);
         push_launch(K_BLUR_V_SMALL, vsj, vs_taps4, vs_rows);
+        push_launch(K_BLUR_V_SMALL8, vs8j, vs8_taps4, vs8_rows);
         push_launch(K_COPY, cj, 0, 0);
         hj.clear(); hj2.clear(); hgj.clear(); hsj.clear(); vj.clear(); vnj.clear(); vgj.clear(); vsj.clear();
+        vs8j.clear();
         cj.clear();
-        vn_rows = hs_taps4 = vs_taps4 = vs_rows = 0;
+        vn_rows = hs_taps4 = vs_taps4 = vs_rows = vs8_taps4 = vs8_rows = 0;
     };
 
     // colour -> grey (to_luma8 / the canvas rule)
@@ -1092,6 +1102,10 @@ int Context::run(const Batch& B, RunOutputs* outs) {
                 case K_BLUR_V_SMALL:
                     launch_blur_v_small((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.max_pad,
                                         L.variant, st);
+                    break;
+                case K_BLUR_V_SMALL8:
+                    launch_blur_v_small8((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.max_pad,
+                                         L.variant, st);
                     break;
                 case K_LUMA: launch_luma((const LumaJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
                 case K_RESIZE_V: launch_resize_v((const ResizeJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
@@ -1977,6 +1991,9 @@ int sift_hip_tune_blur(sift_hip_ctx* ctx, uint32_t width, uint32_t height, float
             case sh::BV_NARROW: sh::launch_blur_v(dv, (int)jobs, vtiles, std::min(h, 16 + 2 * r), C->stream); break;
             case sh::BV_SMALL:
                 sh::launch_blur_v_small(dv, (int)jobs, vtiles, n4, std::min(h, 4 + 2 * r), C->stream);
+                break;
+            case sh::BV_SMALL8:
+                sh::launch_blur_v_small8(dv, (int)jobs, vtiles, n4, std::min(h, 4 + 2 * r), C->stream);
                 break;
             default: sh::launch_blur_v_generic(dv, (int)jobs, vtiles, C->stream); break;
         }

@@ -543,12 +543,15 @@ __device__ __forceinline__ float run_const(float acc, float p, const float* tp, 
 }
 
 // Vertical: the tile's distinct input rows [max(0, y0-r), min(h, y0+4+r)) are
-// staged as f32 (at most h rows). For output row y, taps k < r-y read row 0 and
-// taps k >= h+r-y read row h-1 (clamp to edge): those runs need no loads and
-// their bounds are the same for the whole wave.
-__global__ __launch_bounds__(kThreads) void k_blur_v_small(const BlurJob* __restrict__ jobs,
-                                                           int njobs, int max_taps4) {
-    extern __shared__ __align__(16) float sm[];
+// staged (at most h rows): as f32 (k_blur_v_small, up to kSmallMaxRows rows), or
+// as bytes (k_blur_v_small8: 4x the rows in the same LDS, so levels of a few
+// hundred rows with radii of dozens of pixels, the middle octaves, also get one
+// output per lane; each tap then converts its byte). For output row y, taps
+// k < r-y read row 0 and taps k >= h+r-y read row h-1 (clamp to edge): those
+// runs need no loads and their bounds are the same for the whole wave.
+template <typename T>
+__device__ __forceinline__ void blur_v_small_tile(const BlurJob* __restrict__ jobs, int njobs,
+                                                  int max_taps4, float* sm) {
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const BlurJob& J = jobs[find_job(jobs, njobs, tile)];
     const int t = tile - J.tile_begin;
@@ -556,12 +559,12 @@ __global__ __launch_bounds__(kThreads) void k_blur_v_small(const BlurJob* __rest
     const int w = J.w, h = J.h, n = J.ntaps, r = n >> 1;
     const int n4 = (n + 3) & ~3;
     float* tp = sm;
-    float* col = sm + max_taps4;
+    T* col = reinterpret_cast<T*>(sm + max_taps4);
     for (int i = threadIdx.x; i < n4; i += kThreads) tp[i] = i < n ? J.taps[i] : 0.0f;
     const int rlo = max(y0 - r, 0), rhi = min(y0 + kSTileH + r, h);
     for (int i = threadIdx.x; i < (rhi - rlo) * kSTileW; i += kThreads) {
         const int rr = i >> 6, c = i & 63;
-        col[i] = (float)J.src[(size_t)(rlo + rr) * J.src_pitch + min(x0 + c, w - 1)];
+        col[i] = (T)J.src[(size_t)(rlo + rr) * J.src_pitch + min(x0 + c, w - 1)];
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -570,15 +573,27 @@ __global__ __launch_bounds__(kThreads) void k_blur_v_small(const BlurJob* __rest
     const int ka = min(max(r - y, 0), n);      // taps reading row 0
     const int kc = max(min(h + r - y, n), ka);  // taps from kc on read row h-1
     float acc = 0.0f;
-    if (ka > 0) acc = run_const(acc, col[(0 - rlo) * kSTileW + lane], tp, 0, ka);
-    const float* cp = col + (y - r - rlo) * kSTileW + lane;
+    if (ka > 0) acc = run_const(acc, (float)col[(0 - rlo) * kSTileW + lane], tp, 0, ka);
+    const T* cp = col + (y - r - rlo) * kSTileW + lane;
 #pragma unroll 4
-    for (int k = ka; k < kc; k++) acc = mul_add_exact(acc, cp[k * kSTileW], tp[k]);
-    if (kc < n) acc = run_const(acc, col[(h - 1 - rlo) * kSTileW + lane], tp, kc, n);
+    for (int k = ka; k < kc; k++) acc = mul_add_exact(acc, (float)cp[k * kSTileW], tp[k]);
+    if (kc < n) acc = run_const(acc, (float)col[(h - 1 - rlo) * kSTileW + lane], tp, kc, n);
     const unsigned v = u8_trunc(acc);
     J.dst[(size_t)y * J.dst_pitch + x] = (uint8_t)v;
     if (J.dog)
         J.dog[(size_t)y * J.dog_pitch + x] = q255(v) - q255(J.prev[(size_t)y * J.dst_pitch + x]);
+}
+
+__global__ __launch_bounds__(kThreads) void k_blur_v_small(const BlurJob* __restrict__ jobs,
+                                                           int njobs, int max_taps4) {
+    extern __shared__ __align__(16) float sm[];
+    blur_v_small_tile<float>(jobs, njobs, max_taps4, sm);
+}
+
+__global__ __launch_bounds__(kThreads) void k_blur_v_small8(const BlurJob* __restrict__ jobs,
+                                                            int njobs, int max_taps4) {
+    extern __shared__ __align__(16) float sm8[];
+    blur_v_small_tile<uint8_t>(jobs, njobs, max_taps4, sm8);
 }
 
 // Exact thread-per-pixel passes for kernels too wide for the LDS row tile.
@@ -908,6 +923,14 @@ void launch_blur_v_small(const BlurJob* d_jobs, int njobs, int total_tiles, int 
     if (total_tiles <= 0) return;
     const size_t lds = ((size_t)max_taps4 + (size_t)max_rows * kSTileW) * sizeof(float);
     hipLaunchKernelGGL(k_blur_v_small, dim3(total_tiles), dim3(kThreads), lds, s, d_jobs, njobs,
+                       max_taps4);
+}
+
+void launch_blur_v_small8(const BlurJob* d_jobs, int njobs, int total_tiles, int max_taps4,
+                          int max_rows, hipStream_t s) {
+    if (total_tiles <= 0) return;
+    const size_t lds = (size_t)max_taps4 * sizeof(float) + (size_t)max_rows * kSTileW;
+    hipLaunchKernelGGL(k_blur_v_small8, dim3(total_tiles), dim3(kThreads), lds, s, d_jobs, njobs,
                        max_taps4);
 }
 

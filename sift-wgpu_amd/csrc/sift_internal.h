@@ -132,6 +132,9 @@ void launch_blur_h_small(const BlurJob* d_jobs, int njobs, int total_tiles, int 
                          hipStream_t s);
 void launch_blur_v_small(const BlurJob* d_jobs, int njobs, int total_tiles, int max_taps4,
                          int max_rows, hipStream_t s);
+// Byte-staged form of launch_blur_v_small (max_rows * 64 B of rows instead of f32).
+void launch_blur_v_small8(const BlurJob* d_jobs, int njobs, int total_tiles, int max_taps4,
+                          int max_rows, hipStream_t s);
 void launch_blur_h_generic(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
 // One tile = kLumaTileW pixels of one row.
 void launch_luma(const LumaJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
@@ -180,6 +183,9 @@ constexpr int kMaxHPad = 3576;
 // Small-level kernels: taps (rounded up to 4) and staged rows they accept.
 constexpr int kSmallMaxTaps4 = 2800;
 constexpr int kSmallMaxRows = 128;
+// Staged rows of the byte-staged small vertical form: with kSmallMaxTaps4 taps,
+// 11.2 KB + 50 KB of LDS, within 64 KB whatever jobs share a launch.
+constexpr int kSmallMaxRows8 = 800;
 constexpr int kScanTileElems = 4096;
 // Counters after the extremum row-slot counts, zeroed with them: the
 // descriptor's two lists, then the work-queue counters (kNumXcd each) of

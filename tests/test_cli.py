@@ -1,15 +1,18 @@
 """The command-line front end (sift-wgpu_amd/sift_cli.py) against the reference
 binary's behaviour (src/main.rs:12-140): argument handling, the stdout lines
 bench/benchmark.py:85-97 parses, and Rust's Debug formatting of the values it
-prints. The end-to-end run needs the device and is marked gpu."""
+prints. The end-to-end run needs the device and is marked gpu; it checks the
+CLI's keypoint count and drawing against the oracle's."""
 import os
 
 import numpy as np
 import pytest
 
 import imgen
+import oracle
 import sift_cli
 import sift_hip
+from parity import DESC_L2_TOL, MIN_PR, match_keypoints
 from sift_hip import SiftBackend
 
 
@@ -66,12 +69,18 @@ def test_end_to_end(tmp_path, monkeypatch, capsys):
     assert sift_cli.main([str(path)]) == 0
     out = capsys.readouterr().out.splitlines()
     kps, desc = sift_hip.Sift.default().detect_batch([rgb])[0]
+    # the oracle on the same picture (its own to_luma8, Sift::default()):
+    # the same keypoints, and the same drawing of them
+    p = oracle.params(1.6, 4, 3, 0.5, 0.04, 10.0)
+    okps, odesc = oracle.detect_and_compute(p, oracle.to_luma8(rgb, oracle.RGB8))
+    m = match_keypoints(okps, kps, odesc, desc)
+    assert m["precision"] >= MIN_PR and m["recall"] >= MIN_PR and m["desc_l2_max"] <= DESC_L2_TOL
     assert "Image loaded successfully: 512x384" in out
     assert "Selected SIFT backend: Hip" in out
-    assert f"Found {len(kps)} keypoints." in out
-    assert f"Computed {len(kps)} descriptors." in out
+    assert f"Found {len(okps)} keypoints." in out
+    assert f"Computed {len(okps)} descriptors." in out
     assert any(l.startswith("SIFT processing took: ") for l in out)
     drawn = np.asarray(Image.open(tmp_path / "data" / "output_Hip.png"))
-    assert np.array_equal(drawn, sift_hip.draw_keypoints_to_image(rgb, kps))
+    assert np.array_equal(drawn, oracle.draw_keypoints(rgb, oracle.RGB8, okps, (255, 0, 0)))
     # a backend this library does not provide is an error, not a fallback
     assert sift_cli.main(["--backend", "cpu", str(path)]) == 1

@@ -39,6 +39,26 @@ def test_disc_radius_6_and_cross_of_radius_1():
     assert np.array_equal(sub, sub.T)
 
 
+def test_disc_rows_of_the_midpoint_recurrence():
+    """The radius-6 disc row by row, worked by hand from the midpoint circle
+    recurrence that draw_filled_circle_mut follows (x = 0, y = r, p = 1 - r;
+    while x <= y: fill rows cy +- y over cx +- x and rows cy +- x over cx +- y;
+    x += 1; p < 0 ? p += 2x + 1 : (y -= 1, p += 2(x - y) + 1)):
+    (x, y, p) = (0, 6, -5) (1, 6, -2) (2, 6, 3) (3, 5, 0) (4, 4, 1), so rows
+    0..2 span +-6, row 3 +-5, row 4 +-4, row 5 +-3 and row 6 +-2. A keypoint
+    pointing straight up keeps its line inside the disc."""
+    m = mask(oracle.draw_keypoints(np.zeros((41, 41), np.uint8), oracle.GRAY8,
+                                   kp(20.0, 20.0, -math.pi / 2)), 0)
+    half = {0: 6, 1: 6, 2: 6, 3: 5, 4: 4, 5: 3, 6: 2}
+    for dy in range(-8, 9):
+        xs = np.nonzero(m[20 + dy])[0] - 20
+        if abs(dy) > 6:
+            assert len(xs) == 0, dy
+        else:
+            hw = half[abs(dy)]
+            assert list(xs) == list(range(-hw, hw + 1)), dy
+
+
 def test_line_pixels_follow_the_segment():
     base = np.zeros((60, 60), np.uint8)
     for a in np.linspace(-math.pi, math.pi, 37):
