@@ -368,14 +368,14 @@ enum KernelId {
     K_BLUR_H = 0, K_BLUR_V, K_COPY, K_BLUR_H_GENERIC, K_RESIZE_V, K_RESIZE_H, K_EXTREMA, K_SCAN,
     K_COMPACT, K_REFINE, K_ORIENT, K_DESCRIPTOR, K_FINALIZE, K_BLUR_V_NARROW, K_BLUR_V_GENERIC,
     K_BLUR_H2, K_BLUR_H_SMALL, K_BLUR_V_SMALL, K_LUMA, K_DESC_U8, K_RESIZE_FUSED, K_BLUR_V_SMALL8,
-    K_COUNT
+    K_BLUR_H16, K_COUNT
 };
 // Names as rocprofv3 reports the kernels (template arguments included).
 const char* const kKernelNames[K_COUNT] = {
     "k_blur_h<8>", "k_blur_v", "k_copy_level", "k_blur_h_generic", "k_resize_v", "k_resize_h",
     "k_extrema_tile", "k_scan", "k_compact", "k_refine", "k_orient", "k_descriptor",
     "k_finalize", "k_blur_v_narrow", "k_blur_v_generic", "k_blur_h<2>", "k_blur_h_small",
-    "k_blur_v_small", "k_luma", "k_desc_u8", "k_resize_fused", "k_blur_v_small8"};
+    "k_blur_v_small", "k_luma", "k_desc_u8", "k_resize_fused", "k_blur_v_small8", "k_blur_h<16>"};
 
 
 double blur_job_bytes(const BlurJob& j, int kind) {
@@ -418,7 +418,7 @@ double resize_job_bytes(const ResizeJob& j, int kind) {
 // with the input rows staged in LDS (k_blur_v_narrow); 64 x 4 tiles, one output
 // per lane (k_blur_v_small); one row per work-group (k_blur_v_generic). Every
 // form computes the same exact sums, so any valid choice gives the same level.
-const char* const kBlurHNames[BH_COUNT] = {"h8", "h2", "hsmall", "hgeneric"};
+const char* const kBlurHNames[BH_COUNT] = {"h8", "h2", "hsmall", "hgeneric", "h16"};
 const char* const kBlurVNames[BV_COUNT] = {"v", "vnarrow", "vsmall", "vgeneric", "vsmall8"};
 
 int blur_form_id(const char* name, bool vertical) {
@@ -433,7 +433,7 @@ int blur_form_id(const char* name, bool vertical) {
 // LDS of the 4 staged rows of k_blur_h<px> (padded as it pads them)
 int blur_h_row_lds(int px, int npad) {
     const int hrow = 64 * px + npad + 8;
-    return (hrow + 8 + ((hrow + 8) >> (px == 8 ? 3 : 5)) + 1) * 4 * (int)sizeof(float);
+    return (hrow + 8 + ((hrow + 8) >> (px == 16 ? 4 : px == 8 ? 3 : 5)) + 1) * 4 * (int)sizeof(float);
 }
 
 bool blur_h_valid(int form, int w, int h, int ntaps, int npad) {
@@ -442,6 +442,7 @@ bool blur_h_valid(int form, int w, int h, int ntaps, int npad) {
     const int n4 = (ntaps + 3) & ~3;
     switch (form) {
         case BH_W8: return blur_h_row_lds(8, npad) <= 65536;
+        case BH_W16: return blur_h_row_lds(16, npad) <= 65536;
         case BH_W2: return blur_h_row_lds(2, npad) <= 65536;
         case BH_SMALL: return n4 <= kSmallMaxTaps4;
         case BH_GENERIC: return true;
@@ -463,14 +464,17 @@ bool blur_v_valid(int form, int w, int h, int ntaps) {
 }
 
 // The built-in choice: small levels and long kernels on narrow levels take one
-// output per lane; wide levels the 8-output row tiles and the 4 x 4 column tiles.
+// output per lane; wide levels the 16- or 8-output row tiles (16 when the last
+// 1024-column tile of a row is at least 7/8 full) and the 4 x 4 column tiles.
 std::pair<int, int> blur_heuristic(int w, int h, int ntaps, int npad) {
-    const int px = w >= 384 ? 8 : 2;
+    const bool full16 = w >= 1536 && ((w + 1023) / 1024 * 1024 - w) * 8 <= w;
+    const int px = full16 ? 16 : w >= 384 ? 8 : 2;
     const int n4 = (ntaps + 3) & ~3, r = ntaps / 2;
     const bool small_level = (long long)w * h <= 65536 || r >= 24;
     int hf, vf;
     if (w < 384 && n4 <= kSmallMaxTaps4 && small_level) hf = BH_SMALL;
-    else if (blur_h_row_lds(px, npad) <= 65536) hf = px == 8 ? BH_W8 : BH_W2;
+    else if (blur_h_row_lds(px, npad) <= 65536) hf = px == 16 ? BH_W16 : px == 8 ? BH_W8 : BH_W2;
+    else if (blur_h_row_lds(8, npad) <= 65536) hf = BH_W8;
     else hf = BH_GENERIC;
     if (h <= kSmallMaxRows && n4 <= kSmallMaxTaps4 && (small_level || w < 192 || ntaps > 64)) vf = BV_SMALL;
     else if (w >= 192 && ntaps <= 64) vf = BV_WIDE;
@@ -484,6 +488,7 @@ void blur_h_tiles(BlurJob& J, int form) {
     const int w = J.w, h = J.h;
     switch (form) {
         case BH_W8: J.tiles_x = (w + 511) / 512; J.tiles = J.tiles_x * ((h + 3) / 4); break;
+        case BH_W16: J.tiles_x = (w + 1023) / 1024; J.tiles = J.tiles_x * ((h + 3) / 4); break;
         case BH_W2: J.tiles_x = (w + 127) / 128; J.tiles = J.tiles_x * ((h + 3) / 4); break;
         case BH_SMALL: J.tiles_x = (w + 63) / 64; J.tiles = J.tiles_x * ((h + 3) / 4); break;
         default: J.tiles_x = (w + 255) / 256; J.tiles = J.tiles_x * h; break;
@@ -759,7 +764,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     char* T = B.ext ? B.ext->base : tables.dev;
     auto AP = [&](size_t off) { return A + off; };
 
-    std::vector<BlurJob> hj, hj2, hgj, hsj, vj, vnj, vgj, vsj, vs8j, cj;
+    std::vector<BlurJob> hj, hj2, hj16, hgj, hsj, vj, vnj, vgj, vsj, vs8j, cj;
     const std::pair<int, int> forced = blur_forced();
     int vn_rows = 0, hs_taps4 = 0, vs_taps4 = 0, vs_rows = 0, vs8_taps4 = 0, vs8_rows = 0;
 
@@ -823,6 +828,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         blur_h_tiles(H, form.first);
         switch (form.first) {
             case BH_W8: hj.push_back(H); break;
+            case BH_W16: hj16.push_back(H); break;
             case BH_W2: hj2.push_back(H); break;
             case BH_SMALL:
                 hsj.push_back(H);
@@ -860,9 +866,11 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     };
 
     auto flush_blur = [&]() {
-        int maxpad = 0, maxpad2 = 0;
+        int maxpad = 0, maxpad2 = 0, maxpad16 = 0;
         for (auto& j : hj) maxpad = std::max(maxpad, j.ntaps_pad);
         for (auto& j : hj2) maxpad2 = std::max(maxpad2, j.ntaps_pad);
+        for (auto& j : hj16) maxpad16 = std::max(maxpad16, j.ntaps_pad);
+        push_launch(K_BLUR_H16, hj16, maxpad16, 16);
         push_launch(K_BLUR_H, hj, maxpad, 8);
         push_launch(K_BLUR_H2, hj2, maxpad2, 2);
         push_launch(K_BLUR_H_GENERIC, hgj, 0, 0);
@@ -875,6 +883,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         push_launch(K_COPY, cj, 0, 0);
         hj.clear(); hj2.clear(); hgj.clear(); hsj.clear(); vj.clear(); vnj.clear(); vgj.clear(); vsj.clear();
         vs8j.clear();
+        hj16.clear();
         cj.clear();
         vn_rows = hs_taps4 = vs_taps4 = vs_rows = vs8_taps4 = vs8_rows = 0;
     };
@@ -1083,6 +1092,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             switch (L.kind) {
                 case K_BLUR_H:
                 case K_BLUR_H2:
+                case K_BLUR_H16:
                     launch_blur_h((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.max_pad, L.variant, st);
                     break;
                 case K_BLUR_V: launch_blur_v((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, 0, st); break;
@@ -1980,6 +1990,7 @@ int sift_hip_tune_blur(sift_hip_ctx* ctx, uint32_t width, uint32_t height, float
     auto launch_h = [&]() {
         switch (hf) {
             case sh::BH_W8: sh::launch_blur_h(dh, (int)jobs, htiles, npad, 8, C->stream); break;
+            case sh::BH_W16: sh::launch_blur_h(dh, (int)jobs, htiles, npad, 16, C->stream); break;
             case sh::BH_W2: sh::launch_blur_h(dh, (int)jobs, htiles, npad, 2, C->stream); break;
             case sh::BH_SMALL: sh::launch_blur_h_small(dh, (int)jobs, htiles, n4, C->stream); break;
             default: sh::launch_blur_h_generic(dh, (int)jobs, htiles, C->stream); break;

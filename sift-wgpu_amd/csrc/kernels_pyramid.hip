@@ -72,34 +72,36 @@ __device__ __forceinline__ const ctap_t* as_const_taps(const float* p) {
 // at or before x0-r (d = (x0-r) mod 4 elements early) is staged in LDS as f32
 // with clamp-to-edge indexing (the reference's edge rule), one word of 4 pixels
 // per lane and step, with pad words so that the lanes' reads, PX elements apart,
-// fall in distinct banks: PX = 8 puts one pad word after every 8 elements
-// (element e at e + e/8, lane stride 9 words), PX = 2 one after every 32
-// (e + e/32). Either way a word's 4 pixels land in 4 consecutive slots.
-// For PX = 8 and the tap counts of the default scale schedule (7 to 15 taps) the
-// taps are fully unrolled: the PX + n - 1 window inputs, elements base + d + k,
-// sit at offsets d + k + (d + k)/8 from the lane's first slot (the same for every
-// lane), so they are read once with immediate offsets, one code path per d, and
-// every tap is a scalar register. Otherwise a register window of PX+4 inputs
+// fall in distinct banks: PX = 16 puts one pad word after every 16 elements
+// (element e at e + e/16, lane stride 17 words), PX = 8 one after every 8
+// (e + e/8, lane stride 9 words), PX = 2 one after every 32 (e + e/32). Either
+// way a word's 4 pixels land in 4 consecutive slots.
+// For PX = 8 and 16 and the tap counts of the default scale schedule (7 to 15
+// taps) the taps are fully unrolled: the PX + n - 1 window inputs, elements
+// base + d + k, sit at offsets hpad(d + k) from the lane's first slot (the same
+// for every lane), so they are read once with immediate offsets, one code path
+// per d, and every tap is a scalar register. PX = 16 gives each wave twice the
+// arithmetic per staged row, against which the row's load latency is spread. Otherwise a register window of PX+4 inputs
 // slides over the taps in blocks of 4; taps are zero-padded to a multiple of 4
 // (p * 0 added to a non-negative sum leaves it unchanged), except that a single
 // leftover tap is applied on its own, and are wave-uniform (scalar registers).
 // ---------------------------------------------------------------------------
 template <int PX>
 __host__ __device__ __forceinline__ int hpad(int c) {
-    return PX == 8 ? c + (c >> 3) : c + (c >> 5);
+    return PX == 16 ? c + (c >> 4) : PX == 8 ? c + (c >> 3) : c + (c >> 5);
 }
 
 // NT taps over the window elements base + D + k, k < PX + NT - 1, at
-// p[D + k + (D + k)/8] (p = the slot of element base, base a multiple of 8), in
-// tap order per output; the first product is the first partial sum (0 + p * w0
-// is p * w0: p * w0 is never -0).
+// p[hpad(D + k)] (p = the slot of element base, base a multiple of PX, which is
+// the padding period for PX = 8 and 16), in tap order per output; the first
+// product is the first partial sum (0 + p * w0 is p * w0: p * w0 is never -0).
 template <int PX, int NT, int D>
 __device__ __forceinline__ void hconv_fixed(const float* __restrict__ p, const ctap_t* taps,
                                             float (&acc)[PX]) {
     constexpr int NW = PX + NT - 1;
     float win[NW];
 #pragma unroll
-    for (int k = 0; k < NW; k++) win[k] = p[D + k + ((D + k) >> 3)];
+    for (int k = 0; k < NW; k++) win[k] = p[hpad<PX>(D + k)];
     float wk[NT];
 #pragma unroll
     for (int k = 0; k < NT; k++) wk[k] = taps[k];
@@ -111,14 +113,14 @@ __device__ __forceinline__ void hconv_fixed(const float* __restrict__ p, const c
         for (int q = 0; q < PX; q++) acc[q] = mul_add_exact(acc[q], win[q + k], wk[k]);
 }
 
-template <int NT>
+template <int PX, int NT>
 __device__ __forceinline__ void hconv_fixed_d(int d, const float* __restrict__ p,
-                                              const ctap_t* taps, float (&acc)[8]) {
+                                              const ctap_t* taps, float (&acc)[PX]) {
     switch (d) {
-        case 0: hconv_fixed<8, NT, 0>(p, taps, acc); break;
-        case 1: hconv_fixed<8, NT, 1>(p, taps, acc); break;
-        case 2: hconv_fixed<8, NT, 2>(p, taps, acc); break;
-        default: hconv_fixed<8, NT, 3>(p, taps, acc); break;
+        case 0: hconv_fixed<PX, NT, 0>(p, taps, acc); break;
+        case 1: hconv_fixed<PX, NT, 1>(p, taps, acc); break;
+        case 2: hconv_fixed<PX, NT, 2>(p, taps, acc); break;
+        default: hconv_fixed<PX, NT, 3>(p, taps, acc); break;
     }
 }
 
@@ -203,7 +205,7 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
         dp[2] = (float)((u >> 16) & 0xff);
         dp[3] = (float)(u >> 24);
     };
-    constexpr int KU = PX == 8 ? 3 : 1;  // words per lane of a row of the default kernels
+    constexpr int KU = PX == 16 ? 5 : PX == 8 ? 3 : 1;  // words per lane of a row of the default kernels
     unsigned uw[KU];
 #pragma unroll
     for (int i = 0; i < KU; i++)
@@ -221,15 +223,15 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
     const ctap_t* taps = as_const_taps(J.taps);
     float acc[PX];
     bool fixed = false;
-    if constexpr (PX == 8) {
-        const float* p = row + hpad<8>(base);
+    if constexpr (PX >= 8) {
+        const float* p = row + hpad<PX>(base);
         fixed = true;
         switch (J.ntaps) {
-            case 7: hconv_fixed_d<7>(d, p, taps, acc); break;
-            case 9: hconv_fixed_d<9>(d, p, taps, acc); break;
-            case 11: hconv_fixed_d<11>(d, p, taps, acc); break;
-            case 13: hconv_fixed_d<13>(d, p, taps, acc); break;
-            case 15: hconv_fixed_d<15>(d, p, taps, acc); break;
+            case 7: hconv_fixed_d<PX, 7>(d, p, taps, acc); break;
+            case 9: hconv_fixed_d<PX, 9>(d, p, taps, acc); break;
+            case 11: hconv_fixed_d<PX, 11>(d, p, taps, acc); break;
+            case 13: hconv_fixed_d<PX, 13>(d, p, taps, acc); break;
+            case 15: hconv_fixed_d<PX, 15>(d, p, taps, acc); break;
             default: fixed = false;
         }
     }
@@ -247,6 +249,17 @@ __global__ __launch_bounds__(kThreads) void k_blur_h(const BlurJob* __restrict__
             v.x = res[0] | (res[1] << 8) | (res[2] << 16) | (res[3] << 24);
             v.y = res[4] | (res[5] << 8) | (res[6] << 16) | (res[7] << 24);
             *(uint2*)dst = v;
+            return;
+        }
+    }
+    if constexpr (PX == 16) {
+        if (x + 16 <= w) {
+            uint4 v;
+            v.x = res[0] | (res[1] << 8) | (res[2] << 16) | (res[3] << 24);
+            v.y = res[4] | (res[5] << 8) | (res[6] << 16) | (res[7] << 24);
+            v.z = res[8] | (res[9] << 8) | (res[10] << 16) | (res[11] << 24);
+            v.w = res[12] | (res[13] << 8) | (res[14] << 16) | (res[15] << 24);
+            *(uint4*)dst = v;
             return;
         }
     }
@@ -890,9 +903,11 @@ void launch_blur_h(const BlurJob* d_jobs, int njobs, int total_tiles, int max_pa
                    hipStream_t s) {
     if (total_tiles <= 0) return;
     const int rowlen = 64 * px + max_pad + 8;
-    const size_t lds =
-        (size_t)kHTileH * ((px == 8 ? hpad<8>(rowlen + 8) : hpad<2>(rowlen + 8)) + 1) * sizeof(float);
-    if (px == 8)
+    const int padded = px == 16 ? hpad<16>(rowlen + 8) : px == 8 ? hpad<8>(rowlen + 8) : hpad<2>(rowlen + 8);
+    const size_t lds = (size_t)kHTileH * (padded + 1) * sizeof(float);
+    if (px == 16)
+        hipLaunchKernelGGL(k_blur_h<16>, dim3(total_tiles), dim3(kThreads), lds, s, d_jobs, njobs);
+    else if (px == 8)
         hipLaunchKernelGGL(k_blur_h<8>, dim3(total_tiles), dim3(kThreads), lds, s, d_jobs, njobs);
     else
         hipLaunchKernelGGL(k_blur_h<2>, dim3(total_tiles), dim3(kThreads), lds, s, d_jobs, njobs);

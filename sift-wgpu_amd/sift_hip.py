@@ -552,7 +552,7 @@ class Sift:
 
 
 # blur forms of the tuning table (sift_hip_tune_blur)
-BLUR_H_FORMS = ("h8", "h2", "hsmall", "hgeneric")
+BLUR_H_FORMS = ("h8", "h2", "hsmall", "hgeneric", "h16")
 BLUR_V_FORMS = ("v", "vnarrow", "vsmall", "vgeneric", "vsmall8")
 
 
