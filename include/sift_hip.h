@@ -191,7 +191,7 @@ int sift_hip_profile(sift_hip_ctx* ctx, int enable);
  * by level width, height and tap count. Times one separable blur of sigma on
  * `jobs` levels of width x height in one launch per pass, with the given
  * horizontal form ("h8", "h2", "hsmall", "hgeneric", "h16") and vertical form ("v",
- * "vnarrow", "vsmall", "vgeneric", "vsmall8"; NULL or "" = the built-in choice), and
+ * "vnarrow", "vsmall", "vgeneric", "vsmall8", "v8"; NULL or "" = the built-in choice), and
  * writes the median over `reps` of the horizontal and the vertical pass, in ms,
  * to out_ms[0] and out_ms[1], and the forms used (indices in the two lists
  * above) to out_forms[0] and [1] (may be NULL). SIFT_ERR_INVALID for a form not

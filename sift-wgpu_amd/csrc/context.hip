@@ -368,20 +368,21 @@ enum KernelId {
     K_BLUR_H = 0, K_BLUR_V, K_COPY, K_BLUR_H_GENERIC, K_RESIZE_V, K_RESIZE_H, K_EXTREMA, K_SCAN,
     K_COMPACT, K_REFINE, K_ORIENT, K_DESCRIPTOR, K_FINALIZE, K_BLUR_V_NARROW, K_BLUR_V_GENERIC,
     K_BLUR_H2, K_BLUR_H_SMALL, K_BLUR_V_SMALL, K_LUMA, K_DESC_U8, K_RESIZE_FUSED, K_BLUR_V_SMALL8,
-    K_BLUR_H16, K_COUNT
+    K_BLUR_H16, K_BLUR_V8, K_COUNT
 };
 // Names as rocprofv3 reports the kernels (template arguments included).
 const char* const kKernelNames[K_COUNT] = {
-    "k_blur_h<8>", "k_blur_v", "k_copy_level", "k_blur_h_generic", "k_resize_v", "k_resize_h",
+    "k_blur_h<8>", "k_blur_v<4>", "k_copy_level", "k_blur_h_generic", "k_resize_v", "k_resize_h",
     "k_extrema_tile", "k_scan", "k_compact", "k_refine", "k_orient", "k_descriptor",
     "k_finalize", "k_blur_v_narrow", "k_blur_v_generic", "k_blur_h<2>", "k_blur_h_small",
-    "k_blur_v_small", "k_luma", "k_desc_u8", "k_resize_fused", "k_blur_v_small8", "k_blur_h<16>"};
+    "k_blur_v_small", "k_luma", "k_desc_u8", "k_resize_fused", "k_blur_v_small8", "k_blur_h<16>",
+    "k_blur_v<8>"};
 
 
 double blur_job_bytes(const BlurJob& j, int kind) {
     double a = (double)j.w * j.h;
     if ((kind == K_BLUR_V || kind == K_BLUR_V_NARROW || kind == K_BLUR_V_GENERIC ||
-         kind == K_BLUR_V_SMALL || kind == K_BLUR_V_SMALL8) && j.dog)
+         kind == K_BLUR_V_SMALL || kind == K_BLUR_V_SMALL8 || kind == K_BLUR_V8) && j.dog)
         return a * (2.0 + 1.0 + 4.0);
     if (kind == K_COPY && j.dog) return a * (2.0 + 4.0);
     return a * 2.0;
@@ -419,7 +420,7 @@ double resize_job_bytes(const ResizeJob& j, int kind) {
 // per lane (k_blur_v_small); one row per work-group (k_blur_v_generic). Every
 // form computes the same exact sums, so any valid choice gives the same level.
 const char* const kBlurHNames[BH_COUNT] = {"h8", "h2", "hsmall", "hgeneric", "h16"};
-const char* const kBlurVNames[BV_COUNT] = {"v", "vnarrow", "vsmall", "vgeneric", "vsmall8"};
+const char* const kBlurVNames[BV_COUNT] = {"v", "vnarrow", "vsmall", "vgeneric", "vsmall8", "v8"};
 
 int blur_form_id(const char* name, bool vertical) {
     if (!name) return -1;
@@ -458,6 +459,7 @@ bool blur_v_valid(int form, int w, int h, int ntaps) {
         case BV_NARROW: return std::min(h, 16 + 2 * r) * 64 <= 65536;
         case BV_SMALL: return h <= kSmallMaxRows && n4 <= kSmallMaxTaps4;
         case BV_SMALL8: return n4 <= kSmallMaxTaps4 && std::min(h, 4 + 2 * r) <= kSmallMaxRows8;
+        case BV_WIDE8: return ntaps == 7 || ntaps == 9 || ntaps == 11 || ntaps == 13 || ntaps == 15;
         case BV_GENERIC: return true;
         default: return false;
     }
@@ -498,6 +500,7 @@ void blur_v_tiles(BlurJob& J, int form) {
     const int w = J.w, h = J.h;
     switch (form) {
         case BV_WIDE: J.tiles_x = (w + 255) / 256; J.tiles = J.tiles_x * ((h + 15) / 16); break;
+        case BV_WIDE8: J.tiles_x = (w + 255) / 256; J.tiles = J.tiles_x * ((h + 31) / 32); break;
         case BV_NARROW: J.tiles_x = (w + 63) / 64; J.tiles = J.tiles_x * ((h + 15) / 16); break;
         case BV_SMALL:
         case BV_SMALL8: J.tiles_x = (w + 63) / 64; J.tiles = J.tiles_x * ((h + 3) / 4); break;
@@ -764,7 +767,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     char* T = B.ext ? B.ext->base : tables.dev;
     auto AP = [&](size_t off) { return A + off; };
 
-    std::vector<BlurJob> hj, hj2, hj16, hgj, hsj, vj, vnj, vgj, vsj, vs8j, cj;
+    std::vector<BlurJob> hj, hj2, hj16, hgj, hsj, vj, v8j, vnj, vgj, vsj, vs8j, cj;
     const std::pair<int, int> forced = blur_forced();
     int vn_rows = 0, hs_taps4 = 0, vs_taps4 = 0, vs_rows = 0, vs8_taps4 = 0, vs8_rows = 0;
 
@@ -847,6 +850,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         blur_v_tiles(V, form.second);
         switch (form.second) {
             case BV_WIDE: vj.push_back(V); break;
+            case BV_WIDE8: v8j.push_back(V); break;
             case BV_NARROW:
                 vnj.push_back(V);
                 vn_rows = std::max(vn_rows, std::min(h, 16 + 2 * r));
@@ -876,6 +880,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         push_launch(K_BLUR_H_GENERIC, hgj, 0, 0);
         push_launch(K_BLUR_H_SMALL, hsj, hs_taps4, 0);
         push_launch(K_BLUR_V, vj, 0, 0);
+        push_launch(K_BLUR_V8, v8j, 0, 0);
         push_launch(K_BLUR_V_NARROW, vnj, 0, vn_rows);
         push_launch(K_BLUR_V_GENERIC, vgj, 0, 0);
         push_launch(K_BLUR_V_SMALL, vsj, vs_taps4, vs_rows);
@@ -884,6 +889,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         hj.clear(); hj2.clear(); hgj.clear(); hsj.clear(); vj.clear(); vnj.clear(); vgj.clear(); vsj.clear();
         vs8j.clear();
         hj16.clear();
+        v8j.clear();
         cj.clear();
         vn_rows = hs_taps4 = vs_taps4 = vs_rows = vs8_taps4 = vs8_rows = 0;
     };
@@ -1096,6 +1102,7 @@ int Context::run(const Batch& B, RunOutputs* outs) {
                     launch_blur_h((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.max_pad, L.variant, st);
                     break;
                 case K_BLUR_V: launch_blur_v((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, 0, st); break;
+                case K_BLUR_V8: launch_blur_v8((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, st); break;
                 case K_BLUR_V_NARROW:
                     launch_blur_v((const BlurJob*)(D + L.job_off), L.njobs, L.tiles, L.variant, st);
                     break;
@@ -1999,6 +2006,7 @@ int sift_hip_tune_blur(sift_hip_ctx* ctx, uint32_t width, uint32_t height, float
     auto launch_v = [&]() {
         switch (vf) {
             case sh::BV_WIDE: sh::launch_blur_v(dv, (int)jobs, vtiles, 0, C->stream); break;
+            case sh::BV_WIDE8: sh::launch_blur_v8(dv, (int)jobs, vtiles, C->stream); break;
             case sh::BV_NARROW: sh::launch_blur_v(dv, (int)jobs, vtiles, std::min(h, 16 + 2 * r), C->stream); break;
             case sh::BV_SMALL:
                 sh::launch_blur_v_small(dv, (int)jobs, vtiles, n4, std::min(h, 4 + 2 * r), C->stream);

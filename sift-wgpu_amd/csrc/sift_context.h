@@ -126,7 +126,7 @@ struct Server;
 // height and tap count, and whether its launch carries many jobs (a batch of
 // at least kTuneManyJobs frames) or few.
 enum BlurHForm { BH_W8 = 0, BH_W2, BH_SMALL, BH_GENERIC, BH_W16, BH_COUNT };
-enum BlurVForm { BV_WIDE = 0, BV_NARROW, BV_SMALL, BV_GENERIC, BV_SMALL8, BV_COUNT };
+enum BlurVForm { BV_WIDE = 0, BV_NARROW, BV_SMALL, BV_GENERIC, BV_SMALL8, BV_WIDE8, BV_COUNT };
 constexpr int kTuneManyJobs = 8;
 struct TuneKey {
     int w, h, ntaps, many;

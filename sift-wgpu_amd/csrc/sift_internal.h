@@ -118,6 +118,8 @@ void launch_blur_h(const BlurJob* d_jobs, int njobs, int total_tiles, int max_pa
 // narrow_rows > 0 selects the 64x16-tile LDS variant with that many staged rows.
 void launch_blur_v(const BlurJob* d_jobs, int njobs, int total_tiles, int narrow_rows,
                    hipStream_t s);
+// 8 rows per lane (256 x 32 tiles); only for 7, 9, 11, 13 or 15 taps.
+void launch_blur_v8(const BlurJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
 void launch_resize_v(const ResizeJob* d_jobs, int njobs, int total_tiles, hipStream_t s);
 // Both Lanczos3 passes in one kernel, the f32 intermediate kept in LDS: for jobs
 // whose every tile of kResizeRows x 256 outputs needs at most kResizeFusedSpan

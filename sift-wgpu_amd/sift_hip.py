@@ -553,7 +553,7 @@ class Sift:
 
 # blur forms of the tuning table (sift_hip_tune_blur)
 BLUR_H_FORMS = ("h8", "h2", "hsmall", "hgeneric", "h16")
-BLUR_V_FORMS = ("v", "vnarrow", "vsmall", "vgeneric", "vsmall8")
+BLUR_V_FORMS = ("v", "vnarrow", "vsmall", "vgeneric", "vsmall8", "v8")
 
 
 def tune_blur(width, height, sigma, jobs, h_form=None, v_form=None, reps=10, device=0):
