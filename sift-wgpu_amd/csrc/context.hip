@@ -614,8 +614,8 @@ int Context::run(const Batch& B, RunOutputs* outs) {
     // --- device layout ---
     Carver C;
     if (B.ext) C.off = round_up(tables.host.size(), 256);  // the tables come first
-    std::vector<size_t> in_off(n), htemp_off(n), rtmp_off(n);
-    std::vector<std::vector<size_t>> lev_off(n), dog_off(n);
+    std::vector<size_t> in_off(n), rtmp_off(n);
+    std::vector<std::vector<size_t>> lev_off(n), dog_off(n), htemp_off(n);
     std::vector<int> in_pitch(n);
     // colour inputs: raw pixels (host inputs are staged at color_off) and the
     // grey image k_luma writes at in_off
@@ -642,13 +642,15 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         }
         lev_off[i].resize(shapes[i].size());
         dog_off[i].resize(shapes[i].size());
+        htemp_off[i].resize(shapes[i].size());
         for (size_t o = 0; o < shapes[i].size(); o++) {
             const OctShape& s = shapes[i][o];
             lev_off[i][o] = C.take((size_t)L * s.pitch * s.h + 64);
             if (B.want_dog) dog_off[i][o] = C.take((size_t)(L - 1) * s.w * s.h * sizeof(float));
+            // an H-pass temporary per octave: two octaves of an image can blur in one step
+            htemp_off[i][o] = C.take((size_t)s.pitch * s.h + 64);
         }
         const OctShape& s0 = shapes[i][0];
-        htemp_off[i] = C.take((size_t)s0.pitch * s0.h + 64);
         rtmp_off[i] = C.take((size_t)s0.pitch * std::max(s0.h / 2, 1) * sizeof(float));
     }
 
@@ -741,7 +743,9 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         maxO = std::max(maxO, (int)shapes[i].size());
         Lmax = std::max(Lmax, (int)B.params[i].num_intervals + 3);
     }
-    const size_t max_launches = (size_t)(1 + maxO * Lmax) * 10 + (size_t)maxO * 3 + 8;
+    // per blur step up to 12 blur launches (five H forms, six V forms, the copy)
+    // and 3 Lanczos3 launches; at most 1 + maxO * Lmax steps
+    const size_t max_launches = (size_t)(1 + maxO * Lmax) * 15 + (size_t)maxO * 3 + 8;
     const size_t max_jobs = (size_t)n * ((1 + (size_t)maxO * Lmax) * 2 + (size_t)maxO * 2);
     const size_t blob_bound = max_launches * 256 + max_jobs * std::max(sizeof(BlurJob), sizeof(ResizeJob)) +
                               octs.size() * sizeof(OctaveDesc) + (size_t)n * sizeof(ImageParams) +
@@ -922,32 +926,47 @@ int Context::run(const Batch& B, RunOutputs* outs) {
         const uint8_t* src = B.device_input && B.format(i) == SIFT_PIXEL_GRAY8
                                  ? B.images[i].data
                                  : (const uint8_t*)AP(in_off[i]);
-        make_blur_pair(src, in_pitch[i], (uint8_t*)AP(htemp_off[i]), s.pitch,
+        make_blur_pair(src, in_pitch[i], (uint8_t*)AP(htemp_off[i][0]), s.pitch,
                        (uint8_t*)AP(lev_off[i][0]), s.pitch, s.w, s.h, ptab[i]->initial, nullptr,
                        nullptr, 0);
     }
     flush_blur();
-    for (int o = 0; o < maxO; o++) {
-        for (int s = 1; s < Lmax; s++) {
-            for (int i = 0; i < n; i++) {
-                if ((int)shapes[i].size() <= o) continue;
-                const int L = (int)B.params[i].num_intervals + 3;
-                if (s >= L) continue;
+    // The levels as a wavefront: level s >= 1 of octave o of image i is blurred in
+    // step o * S_i + s. Octave o + 1 needs only level S of octave o (its Lanczos3
+    // base, made at the end of step (o + 1) * S_i), so its first levels share
+    // launches with levels S + 1, S + 2 of octave o: a 5-octave, S = 3 frame takes
+    // 18 blur steps instead of 26. Every level still reads the finished level
+    // before it; only the order of independent launches changes.
+    int steps = 0;
+    for (int i = 0; i < n; i++) {
+        const int S = (int)B.params[i].num_intervals;
+        steps = std::max(steps, ((int)shapes[i].size() - 1) * S + S + 2);
+    }
+    for (int t = 1; t <= steps; t++) {
+        for (int i = 0; i < n; i++) {
+            const int S = (int)B.params[i].num_intervals, L = S + 3;
+            for (int o = 0; o < (int)shapes[i].size(); o++) {
+                const int s = t - o * S;
+                if (s < 1 || s >= L) continue;
                 const OctShape& sh = shapes[i][o];
                 const size_t lsz = (size_t)sh.pitch * sh.h;
                 uint8_t* base = (uint8_t*)AP(lev_off[i][o]);
                 float* dog = B.want_dog ? (float*)AP(dog_off[i][o]) + (size_t)(s - 1) * sh.w * sh.h : nullptr;
-                make_blur_pair(base + (s - 1) * lsz, sh.pitch, (uint8_t*)AP(htemp_off[i]), sh.pitch,
+                make_blur_pair(base + (s - 1) * lsz, sh.pitch, (uint8_t*)AP(htemp_off[i][o]), sh.pitch,
                                base + s * lsz, sh.pitch, sh.w, sh.h, ptab[i]->level[(size_t)o * L + s],
                                dog, base + (s - 1) * lsz, sh.w);
             }
-            flush_blur();
         }
-        // Lanczos3 to the next octave base (src/sift.rs:166-173)
+        flush_blur();
+        // Lanczos3 to the next octave base (src/sift.rs:166-173), for every octave
+        // whose level S was made in this step
         std::vector<ResizeJob> rv, rh, rf;
         const bool fuse_resize = resize_fusion_enabled();
         for (int i = 0; i < n; i++) {
-            if ((int)shapes[i].size() <= o + 1) continue;
+            const int Si = (int)B.params[i].num_intervals;
+            if (t % Si != 0) continue;
+            const int o = t / Si - 1;
+            if (o < 0 || (int)shapes[i].size() <= o + 1) continue;
             const OctShape& a = shapes[i][o];
             const OctShape& b = shapes[i][o + 1];
             if (a.w <= 0 || a.h <= 0 || b.w <= 0 || b.h <= 0) continue;

@@ -55,6 +55,7 @@ size_t frame_device_bytes(uint32_t w, uint32_t h, const sift_params_t& p, int bp
     uint32_t ww = w, hh = h;
     for (uint32_t o = 0; o < p.num_octaves; o++) {
         bytes += L * round_up_sz(ww ? ww : 1, 64) * hh + 64 + kAlign;  // levels
+        bytes += round_up_sz(ww ? ww : 1, 64) * hh + 64 + kAlign;      // its H-pass temporary
         px_layers += S * ww * hh;
         slots += S * hh;
         words += S * hh * ((ww + 63) / 64);
@@ -62,7 +63,6 @@ size_t frame_device_bytes(uint32_t w, uint32_t h, const sift_params_t& p, int bp
         hh /= 2;
     }
     const size_t pitch0 = round_up_sz(w, 64);
-    bytes += pitch0 * h + 64 + kAlign;                                   // H-pass temporary
     bytes += pitch0 * std::max<size_t>(h / 2, 1) * 4 + kAlign;           // Lanczos temporary
     bytes += words * 8 + 2 * (slots + kCountExtra) * 4 + (slots / 4096 + 4) * 4 + 4 * kAlign;  // bitmask, counts, scan
     const size_t cand = std::max<size_t>(4096, px_layers / 128);
@@ -76,7 +76,7 @@ size_t frame_device_bytes(uint32_t w, uint32_t h, const sift_params_t& p, int bp
     const size_t O = p.num_octaves;
     bytes += ((1 + O * L) * 2 + O * 2) * std::max(sizeof(BlurJob), sizeof(ResizeJob)) +
              O * sizeof(OctaveDesc) + sizeof(ImageParams) + 16;
-    bytes += ((1 + O * L) * 10 + O * 3 + 8) * 256 + 4096 + 64 + kAlign;
+    bytes += ((1 + O * L) * 15 + O * 3 + 8) * 256 + 4096 + 64 + kAlign;
     return bytes;
 }
 
