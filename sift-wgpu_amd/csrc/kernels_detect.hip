@@ -227,6 +227,7 @@ constexpr int kTileW = 64, kTileH = 16;
 static_assert(kExtStripRows % kTileH == 0, "strips are whole bands");
 constexpr int kExtNW = (kTileW + 16) / 8;         // 8-byte words covering [x0-8, x0+72)
 constexpr int kExtItems = (kTileH + 2) * kExtNW;  // (row, word) items of a band
+constexpr int kExtRW = kTileW + 8;                // staged D row: columns [x0-4, x0+68)
 static_assert(kExtItems <= kThreads, "one item per thread");
 typedef __attribute__((address_space(1))) const uint8_t gbyte_t;
 typedef __attribute__((address_space(1))) const unsigned long long gu64_t;
@@ -264,26 +265,37 @@ __device__ __forceinline__ void ext_fetch(const OctaveDesc& O, int x0, int y0,
     }
 }
 
-// D = q[G_{l+1}] - q[G_l] for columns [x0-1, x0+65) of the band's 18 rows, with
-// q formed by q255f, equal to v / 255.0f for every byte value; D is the same f32
-// subtraction the reference makes.
+// D = q[G_{l+1}] - q[G_l] for columns [x0-4, x0+68) of the band's 18 rows (the
+// test reads [x0-1, x0+65)), with q formed by q255f, equal to v / 255.0f for
+// every byte value; D is the same f32 subtraction the reference makes. Column
+// x0-4+j sits at D index j, so the 4-column halves of a thread's 8-byte word
+// (columns x0-8+8k ..) start at 16-byte boundaries and each goes to LDS as one
+// 16-byte write (the first half of word 0 and the second half of the last word
+// lie outside the staged columns).
 template <int NL, int NG>
 __device__ __forceinline__ void ext_stage(const unsigned long long (&u)[NG],
-                                          float (*D)[kTileH + 2][kTileW + 2]) {
+                                          float (*D)[kTileH + 2][kExtRW]) {
     const int i = threadIdx.x;
     if (i >= kExtItems) return;
     const int r = i / kExtNW, k = i - r * kExtNW;
 #pragma unroll
-    for (int b = 0; b < 8; b++) {
-        const int c = 8 * k + b - 7;  // column x0-1+c
-        if (c >= 0 && c < kTileW + 2) {
-            float qlo = q255f((float)(unsigned)(u[0] >> (8 * b) & 0xffu));
+    for (int hh = 0; hh < 2; hh++) {
+        if (hh == 0 ? k == 0 : k == kExtNW - 1) continue;
+        float* const dst = &D[0][r][8 * k + 4 * hh - 4];
+        const unsigned w0 = (unsigned)(u[0] >> (32 * hh));
+        float qlo[4];
 #pragma unroll
-            for (int l = 0; l < NL; l++) {
-                const float qhi = q255f((float)(unsigned)(u[l + 1] >> (8 * b) & 0xffu));
-                D[l][r][c] = qhi - qlo;
-                qlo = qhi;
-            }
+        for (int b = 0; b < 4; b++) qlo[b] = q255f((float)((w0 >> (8 * b)) & 0xffu));
+#pragma unroll
+        for (int l = 0; l < NL; l++) {
+            const unsigned w1 = (unsigned)(u[l + 1] >> (32 * hh));
+            float qhi[4];
+#pragma unroll
+            for (int b = 0; b < 4; b++) qhi[b] = q255f((float)((w1 >> (8 * b)) & 0xffu));
+            *reinterpret_cast<float4*>(dst + l * (kTileH + 2) * kExtRW) =
+                make_float4(qhi[0] - qlo[0], qhi[1] - qlo[1], qhi[2] - qlo[2], qhi[3] - qlo[3]);
+#pragma unroll
+            for (int b = 0; b < 4; b++) qlo[b] = qhi[b];
         }
     }
 }
@@ -296,8 +308,7 @@ void k_extrema_tile(const OctaveDesc* __restrict__ oct, int noct,
                     unsigned* __restrict__ counts, int strip_rows) {
     constexpr int NL = S + 2;  // DoG layers
     constexpr int NG = S + 3;  // Gaussian levels
-    constexpr int RW = kTileW + 2;
-    __shared__ float D[NL][kTileH + 2][RW];
+    __shared__ __align__(16) float D[NL][kTileH + 2][kExtRW];
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int j = find_by(oct, noct, tile, &OctaveDesc::row_begin);
     const OctaveDesc& O = oct[j];
@@ -325,9 +336,9 @@ void k_extrema_tile(const OctaveDesc* __restrict__ oct, int noct,
         auto load_row = [&](int slot, int r) {
 #pragma unroll
             for (int l = 0; l < NL; l++) {
-                const float a = D[l][r][lane];
-                const float b = D[l][r][lane + 1];
-                const float c = D[l][r][lane + 2];
+                const float a = D[l][r][lane + 3];  // column x0 + lane - 1
+                const float b = D[l][r][lane + 4];
+                const float c = D[l][r][lane + 5];
                 dl[slot][l] = a;
                 dc[slot][l] = b;
                 dr[slot][l] = c;
