@@ -2,12 +2,14 @@
 //
 // sift_hip_submit copies a frame into host memory and queues it. Worker lanes
 // (four by default, fewer when the HBM budget is small), each with its own HIP
-// stream and device arena, take every queued frame that fits (up to max_batch,
-// and while the device bytes of the lane's batch stay within its share of the
-// HBM budget), run them as one batch through the same kernels as
+// stream and device arena, take queued frames (up to max_batch, while the device
+// bytes of the lane's batch stay within its share of the HBM budget, and up to
+// about two 1080p frames of pixels, or an equal share of a longer queue per
+// lane), run them as one batch through the same kernels as
 // sift_hip_detect_batch, and post completions. Frames that arrive while batches
 // are on the device go into the next batch of whichever lane frees first, so the
-// lanes' batches overlap on the device and queueing stays short under load.
+// lanes' batches overlap on the device and queueing stays short under load; a
+// backlog is taken in fewer, larger batches, which raise the served rate.
 //
 // Device memory: each lane's arena is allocated once, at start, as its share
 // (budget / lanes) and never grows past it, so the lanes together hold at most
@@ -40,8 +42,8 @@ size_t round_up_sz(size_t v, size_t a) { return (v + a - 1) / a * a; }
 constexpr int kDefaultLanes = 4;
 // A lane per this many bytes of budget, up to kDefaultLanes.
 constexpr size_t kLaneBytes = (size_t)2 << 30;
-// Pixels of one device batch (about two 1080p frames); a larger first frame
-// still runs alone.
+// Pixels of one device batch while the queue is short (about two 1080p
+// frames); a larger first frame still runs alone.
 constexpr size_t kMaxBatchPixels = (size_t)4200000;
 
 }  // namespace
@@ -85,13 +87,19 @@ struct Request {
     uint32_t w, h;
     uint32_t format;  // SIFT_PIXEL_*
     sift_params_t params;
-    std::vector<uint8_t> pixels;
+    std::unique_ptr<uint8_t[]> pixels;  // the frame's rows, packed (not zero-filled first)
     double submit_ms;
     size_t bytes;
 };
 
 struct Server {
     bool trace = false;  // SIFT_HIP_TRACE=1: one stderr line per batch
+    // Pixels of one batch: base_px (SIFT_HIP_BATCH_PIXELS overrides it), or,
+    // when more is queued, an equal share of the queue per lane
+    // (SIFT_HIP_BATCH_ADAPT=0 turns that off).
+    size_t base_px = kMaxBatchPixels;
+    bool adapt = true;
+    size_t pending_px = 0;  // pixels of the queued frames
     Context* owner = nullptr;
     sift_server_config_t cfg{};
     size_t lane_share = 0;  // device bytes of one lane's arena
@@ -119,7 +127,7 @@ void run_batch(Context* lane, const std::vector<Request>& batch, std::vector<sif
         Batch B;
         for (size_t i = i0; i < i1; i++) {
             const uint32_t row = batch[i].w * (uint32_t)pixel_bytes(batch[i].format);
-            B.images.push_back(sift_image_t{batch[i].pixels.data(), batch[i].w, batch[i].h, row});
+            B.images.push_back(sift_image_t{batch[i].pixels.get(), batch[i].w, batch[i].h, row});
             B.formats.push_back(batch[i].format);
             B.params.push_back(batch[i].params);
         }
@@ -143,14 +151,18 @@ void Server::worker(Context* lane) {
             for (;;) {
                 if (stopping && pending.empty()) return;
                 size_t batch_px = 0;
+                // small batches while the queue is short (latency); under a
+                // backlog each lane takes its share of it (fewer, larger batches),
+                // so the backlog still spreads over the lanes
+                const size_t cap_px = adapt ? std::max(base_px, pending_px / lanes.size()) : base_px;
                 while (!pending.empty() && batch.size() < cfg.max_batch) {
                     Request& r = pending.front();
                     if (bytes_in_flight + r.bytes > cfg.hbm_budget_bytes) break;
                     if (batch_bytes + r.bytes > lane_share) break;
-                    // keep batches small enough that a backlog spreads over the lanes
                     const size_t px = (size_t)r.w * r.h;
-                    if (!batch.empty() && batch_px + px > kMaxBatchPixels) break;
+                    if (!batch.empty() && batch_px + px > cap_px) break;
                     batch_px += px;
+                    pending_px -= px;
                     bytes_in_flight += r.bytes;
                     batch_bytes += r.bytes;
                     batch.push_back(std::move(r));
@@ -240,15 +252,23 @@ int submit_frame(sift_hip_ctx* ctx, const sift_color_image_t* image, const sift_
         rc = set_error(SIFT_ERR_NOMEM, "frame needs more device memory than a lane's share of the HBM budget");
     } else {
         const size_t row = (size_t)r.w * bpp;
-        r.pixels.resize(row * r.h);
+        r.pixels.reset(new (std::nothrow) uint8_t[row * r.h]);
         const size_t stride = image->stride ? image->stride : row;
-        for (uint32_t y = 0; y < r.h; y++)
-            memcpy(r.pixels.data() + (size_t)y * row, image->data + (size_t)y * stride, row);
+        if (r.pixels && stride == row) {
+            memcpy(r.pixels.get(), image->data, row * r.h);
+        } else if (r.pixels) {
+            for (uint32_t y = 0; y < r.h; y++)
+                memcpy(r.pixels.get() + (size_t)y * row, image->data + (size_t)y * stride, row);
+        }
         r.submit_ms = t_submit;
         std::lock_guard<std::mutex> lk(S->mu);
-        if (S->stopping) rc = set_error(SIFT_ERR_INVALID, "server is stopping");
+        if (!r.pixels) rc = set_error(SIFT_ERR_NOMEM, "host allocation failed");
+        else if (S->stopping) rc = set_error(SIFT_ERR_INVALID, "server is stopping");
         else if (S->pending.size() >= S->cfg.max_queue) rc = set_error(SIFT_ERR_BUSY, "server queue is full");
-        else S->pending.push_back(std::move(r));
+        else {
+            S->pending_px += (size_t)r.w * r.h;
+            S->pending.push_back(std::move(r));
+        }
     }
     if (rc == SIFT_OK) S->cv_work.notify_one();
     release_server(C, S);
@@ -274,6 +294,8 @@ int sift_hip_server_start(sift_hip_ctx* ctx, const sift_server_config_t* cfg) {
     S->cfg = *cfg;
     const char* tr = getenv("SIFT_HIP_TRACE");
     S->trace = tr && tr[0] == '1';
+    if (const char* e = getenv("SIFT_HIP_BATCH_PIXELS")) S->base_px = (size_t)std::max(1L, atol(e));
+    if (const char* e = getenv("SIFT_HIP_BATCH_ADAPT")) S->adapt = e[0] != '0';
     if (S->cfg.max_batch == 0) S->cfg.max_batch = 64;
     if (S->cfg.max_queue == 0) S->cfg.max_queue = 4096;
     if (S->cfg.hbm_budget_bytes == 0) S->cfg.hbm_budget_bytes = (size_t)32 << 30;

@@ -313,30 +313,40 @@ def wgpu_descriptor_view(desc_u8):
     return np.asarray(desc_u8, np.uint8).astype(np.float32) / np.float32(255.0)
 
 
-def _take_result_u8(r):
+class _ResultOwner:
+    """Owns a result's C buffers (malloc'd by the library) and frees them once
+    the last array over them is gone."""
+
+    def __init__(self, r, free):
+        self._r, self._free = r, free
+
+    def __del__(self):
+        self._free(ctypes.byref(self._r))
+
+
+def _result_arrays(r, free, dtype):
+    """(keypoints, descriptors) of a sift_result_t / sift_result_u8_t as numpy
+    arrays over the library's own buffers (no copy); the buffers are released by
+    `free` when both arrays and every view of them are gone."""
     n = int(r.count)
-    if n:
-        kb = (ctypes.c_char * (n * KEYPOINT_DTYPE.itemsize)).from_address(r.keypoints)
-        kps = np.frombuffer(bytes(kb), KEYPOINT_DTYPE).copy()
-        desc = np.ctypeslib.as_array(r.descriptors, shape=(n * 128,)).copy().reshape(n, 128)
-    else:
-        kps = np.zeros(0, KEYPOINT_DTYPE)
-        desc = np.zeros((0, 128), np.uint8)
-    lib().sift_result_u8_free(ctypes.byref(r))
-    return kps, desc
+    if n == 0:
+        free(ctypes.byref(r))
+        return np.zeros(0, KEYPOINT_DTYPE), np.zeros((0, 128), dtype)
+    owner = _ResultOwner(type(r).from_buffer_copy(r), free)
+    kb = (ctypes.c_char * (n * KEYPOINT_DTYPE.itemsize)).from_address(r.keypoints)
+    db = (ctypes.c_char * (n * 128 * np.dtype(dtype).itemsize)).from_address(
+        ctypes.cast(r.descriptors, ctypes.c_void_p).value)
+    kb._owner = owner
+    db._owner = owner
+    return np.frombuffer(kb, KEYPOINT_DTYPE), np.frombuffer(db, dtype).reshape(n, 128)
+
+
+def _take_result_u8(r):
+    return _result_arrays(r, lib().sift_result_u8_free, np.uint8)
 
 
 def _take_result(r):
-    n = int(r.count)
-    if n:
-        kb = (ctypes.c_char * (n * KEYPOINT_DTYPE.itemsize)).from_address(r.keypoints)
-        kps = np.frombuffer(bytes(kb), KEYPOINT_DTYPE).copy()
-        desc = np.ctypeslib.as_array(r.descriptors, shape=(n * 128,)).copy().reshape(n, 128)
-    else:
-        kps = np.zeros(0, KEYPOINT_DTYPE)
-        desc = np.zeros((0, 128), np.float32)
-    lib().sift_result_free(ctypes.byref(r))
-    return kps, desc
+    return _result_arrays(r, lib().sift_result_free, np.float32)
 
 
 def _take_kps(ptr, n):

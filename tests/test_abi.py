@@ -84,3 +84,42 @@ def test_create_without_device_reports_device_error():
     out = ctypes.c_void_p()
     assert L.sift_hip_create(ctypes.byref(p), 0, ctypes.byref(out)) == -2
     assert "device" in sift_hip.last_error()
+
+
+def test_result_arrays_view_the_library_buffers_and_free_them_once():
+    """Results reach Python as arrays over the library's malloc'd buffers (no
+    copy); sift_result_free runs once, after the last array or view is gone."""
+    import gc
+    libc = ctypes.CDLL(None)
+    libc.malloc.restype = ctypes.c_void_p
+    libc.malloc.argtypes = [ctypes.c_size_t]
+    n = 3
+    kps = np.zeros(n, sift_hip.KEYPOINT_DTYPE)
+    kps["x"] = [1.5, 2.5, 3.5]
+    kps["layer"] = [1, 2, 3]
+    desc = np.arange(n * 128, dtype=np.float32).reshape(n, 128) / 7.0
+    kp_mem, de_mem = libc.malloc(kps.nbytes), libc.malloc(desc.nbytes)
+    ctypes.memmove(kp_mem, kps.ctypes.data, kps.nbytes)
+    ctypes.memmove(de_mem, desc.ctypes.data, desc.nbytes)
+    r = sift_hip.SiftResult(kp_mem, ctypes.cast(de_mem, ctypes.POINTER(ctypes.c_float)), n)
+    freed = []
+
+    def free(p):
+        freed.append(1)
+        sift_hip.lib().sift_result_free(p)
+
+    k, d = sift_hip._result_arrays(r, free, np.float32)
+    assert k.ctypes.data == kp_mem and d.ctypes.data == de_mem  # views, not copies
+    assert np.array_equal(k, kps) and np.array_equal(d, desc)
+    row = d[1]
+    del k, d
+    gc.collect()
+    assert freed == []  # a view of the descriptors is still alive
+    assert np.array_equal(row, desc[1])
+    del row
+    gc.collect()
+    assert freed == [1]
+    # an empty result is released at once and gives empty arrays
+    e = sift_hip.SiftResult(None, None, 0)
+    k, d = sift_hip._result_arrays(e, free, np.float32)
+    assert len(k) == 0 and d.shape == (0, 128) and freed == [1, 1]
