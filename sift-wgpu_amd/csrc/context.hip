@@ -1060,9 +1060,10 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             HIP_TRY(hipMemcpyAsync(B.ext->base, ext_pinned, tables.host.size(), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(A + blob_dev_off, ext_pinned + tb, blob.size(), hipMemcpyHostToDevice, st));
     }
-    // staging: inputs + blob in pinned memory
+    // staging: inputs + blob in pinned memory (inputs that are pinned already
+    // are copied from where they are)
     size_t in_bytes = 0;
-    if (!B.device_input)
+    if (!B.device_input && !B.host_pinned)
         for (int i = 0; i < n; i++)
             in_bytes += (size_t)B.images[i].width * B.images[i].height * pixel_bytes(B.format(i));
     if (!B.ext) {
@@ -1079,15 +1080,21 @@ int Context::run(const Batch& B, RunOutputs* outs) {
             const uint32_t fmt = B.format(i);
             const size_t row = (size_t)im.width * pixel_bytes(fmt);
             const size_t stride = im.stride ? im.stride : row;
-            for (uint32_t y = 0; y < im.height; y++)
-                memcpy(pinned + po + (size_t)y * row, im.data + (size_t)y * stride, row);
+            const uint8_t* src = im.data;
+            size_t spitch = stride;
+            if (!B.host_pinned) {
+                for (uint32_t y = 0; y < im.height; y++)
+                    memcpy(pinned + po + (size_t)y * row, im.data + (size_t)y * stride, row);
+                src = (const uint8_t*)pinned + po;
+                spitch = row;
+                po += row * im.height;
+            }
             if (fmt == SIFT_PIXEL_GRAY8)
-                HIP_TRY(hipMemcpy2DAsync(A + in_off[i], in_pitch[i], pinned + po, row, row, im.height,
+                HIP_TRY(hipMemcpy2DAsync(A + in_off[i], in_pitch[i], src, spitch, row, im.height,
                                          hipMemcpyHostToDevice, st));
             else
-                HIP_TRY(hipMemcpy2DAsync(A + color_off[i], color_pitch[i], pinned + po, row, row,
+                HIP_TRY(hipMemcpy2DAsync(A + color_off[i], color_pitch[i], src, spitch, row,
                                          im.height, hipMemcpyHostToDevice, st));
-            po += row * im.height;
         }
     }
     char* D = A + blob_dev_off;
@@ -1439,6 +1446,64 @@ int check_color_image(const sift_color_image_t* im) {
 
 // Copies the results out: f32 descriptors into `out`, or (when out8 is set)
 // the u8 descriptors into `out8`, with the keypoint convention kp_conv.
+// The f32 results of a finished run (summary `sum` read): each image's
+// keypoints and descriptors in one pinned pool block, [keypoints | pad |
+// descriptors], filled by two DMAs from the device; images the pool cannot
+// serve get malloc'd buffers, filled through the context's pinned staging.
+int collect_pinned(sh::Context* C, int n, sh::RunOutputs& R, const std::vector<unsigned>& sum,
+                   sift_result_t* out) {
+    constexpr size_t kDesc = 128 * sizeof(float);
+    std::vector<char> pooled(n, 0);
+    size_t staged = 0;  // keypoints of the images that go through the staging
+    for (int i = 0; i < n; i++) {
+        const size_t c = sum[3 + i] - sum[2 + i];
+        out[i] = sift_result_t{nullptr, nullptr, c};
+        if (!c) continue;
+        const size_t kb = (c * sizeof(sift_keypoint_t) + 255) / 256 * 256;
+        char* blk = (char*)sh::pinned_alloc(kb + c * kDesc);
+        if (blk) {
+            out[i].keypoints = (sift_keypoint_t*)blk;
+            out[i].descriptors = (float*)(blk + kb);
+            pooled[i] = 1;
+            continue;
+        }
+        out[i].keypoints = (sift_keypoint_t*)malloc(c * sizeof(sift_keypoint_t));
+        out[i].descriptors = (float*)malloc(c * kDesc);
+        if (!out[i].keypoints || !out[i].descriptors) return fail(SIFT_ERR_NOMEM, "host allocation failed");
+        staged += c;
+    }
+    // staging slots of the malloc'd images, in image order
+    sh::Keypoint* skps = nullptr;
+    char* sdesc = nullptr;
+    if (staged) {
+        const size_t kbytes = staged * sizeof(sh::Keypoint);
+        int rc = C->ensure_pinned_out(kbytes + staged * kDesc + 256);
+        if (rc) return rc;
+        skps = (sh::Keypoint*)C->pinned_out;
+        sdesc = C->pinned_out + (kbytes + 255) / 256 * 256;
+    }
+    size_t so = 0;
+    for (int i = 0; i < n; i++) {
+        const size_t b = sum[2 + i], c = out[i].count;
+        if (!c) continue;
+        void* kd = pooled[i] ? (void*)out[i].keypoints : (void*)(skps + so);
+        void* dd = pooled[i] ? (void*)out[i].descriptors : (void*)(sdesc + so * kDesc);
+        HIP_TRY(hipMemcpyAsync(kd, R.d_kps + b, c * sizeof(sh::Keypoint), hipMemcpyDeviceToHost, C->stream));
+        HIP_TRY(hipMemcpyAsync(dd, R.d_desc + b * 128, c * kDesc, hipMemcpyDeviceToHost, C->stream));
+        if (!pooled[i]) so += c;
+    }
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    so = 0;
+    for (int i = 0; i < n; i++) {
+        const size_t c = out[i].count;
+        if (!c || pooled[i]) continue;
+        memcpy(out[i].keypoints, skps + so, c * sizeof(sift_keypoint_t));
+        memcpy(out[i].descriptors, sdesc + so * kDesc, c * kDesc);
+        so += c;
+    }
+    return SIFT_OK;
+}
+
 int collect(sh::Context* C, int n, sh::RunOutputs& R, sift_result_t* out, bool* retry,
             sift_result_u8_t* out8 = nullptr, uint32_t kp_conv = SIFT_KEYPOINTS_CPU) {
     std::vector<unsigned> sum(n + 3);
@@ -1453,6 +1518,7 @@ int collect(sh::Context* C, int n, sh::RunOutputs& R, sift_result_t* out, bool* 
         return SIFT_OK;
     }
     if (!out && !out8) return SIFT_OK;
+    if (!out8) return collect_pinned(C, n, R, sum, out);
     const size_t dsz = out8 ? 128 : 128 * 4;
     const size_t kbytes = nkp * sizeof(sh::Keypoint), dbytes = nkp * dsz;
     int rc = C->ensure_pinned_out(kbytes + dbytes + 256);
@@ -1833,8 +1899,11 @@ int sift_hip_detect_batch_device_noalloc(sift_hip_ctx* ctx, const sift_image_t* 
 
 void sift_result_free(sift_result_t* r) {
     if (!r) return;
-    free(r->keypoints);
-    free(r->descriptors);
+    // one pinned pool block holds both arrays, or two malloc'd buffers
+    if (!sh::pinned_release(r->keypoints)) {
+        free(r->keypoints);
+        free(r->descriptors);
+    }
     r->keypoints = nullptr;
     r->descriptors = nullptr;
     r->count = 0;

@@ -20,8 +20,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kHTileH = 4;    // one row per wave
 constexpr int kVTileW = 256;  // 64 lanes x 4 px
-constexpr int kVRows = 4;     // rows per lane in k_blur_v
-constexpr int kVTileH = 4 * kVRows;  // 4 waves x kVRows rows
+constexpr int kVRows = 4;     // rows per lane in k_blur_v (a tile: 4 waves x kVRows rows)
 
 // The job of a tile (jobs[j].tile_begin <= tile < jobs[j+1].tile_begin); every
 // launch's grid is its total tile count. With more than 4 jobs the proportional

@@ -87,7 +87,31 @@ struct Request {
     uint32_t w, h;
     uint32_t format;  // SIFT_PIXEL_*
     sift_params_t params;
-    std::unique_ptr<uint8_t[]> pixels;  // the frame's rows, packed (not zero-filled first)
+    // the frame's rows, packed: a pinned pool block (copied to the device without
+    // staging), or, when the pool has none to give, plain host memory
+    struct Pixels {
+        uint8_t* p = nullptr;
+        bool pinned = false;
+        Pixels() = default;
+        Pixels(const Pixels&) = delete;
+        Pixels& operator=(const Pixels&) = delete;
+        Pixels(Pixels&& o) noexcept : p(o.p), pinned(o.pinned) { o.p = nullptr; }
+        Pixels& operator=(Pixels&& o) noexcept {
+            std::swap(p, o.p);
+            std::swap(pinned, o.pinned);
+            return *this;
+        }
+        ~Pixels() {
+            if (pinned) pinned_release(p);
+            else delete[] p;
+        }
+        bool alloc(size_t bytes) {
+            p = (uint8_t*)pinned_alloc(bytes);
+            pinned = p != nullptr;
+            if (!p) p = new (std::nothrow) uint8_t[bytes];
+            return p != nullptr;
+        }
+    } pixels;
     double submit_ms;
     size_t bytes;
 };
@@ -125,9 +149,11 @@ void run_batch(Context* lane, const std::vector<Request>& batch, std::vector<sif
                std::vector<int>& rcs) {
     auto make = [&](size_t i0, size_t i1) {
         Batch B;
+        B.host_pinned = true;
         for (size_t i = i0; i < i1; i++) {
             const uint32_t row = batch[i].w * (uint32_t)pixel_bytes(batch[i].format);
-            B.images.push_back(sift_image_t{batch[i].pixels.get(), batch[i].w, batch[i].h, row});
+            B.host_pinned = B.host_pinned && batch[i].pixels.pinned;
+            B.images.push_back(sift_image_t{batch[i].pixels.p, batch[i].w, batch[i].h, row});
             B.formats.push_back(batch[i].format);
             B.params.push_back(batch[i].params);
         }
@@ -252,17 +278,17 @@ int submit_frame(sift_hip_ctx* ctx, const sift_color_image_t* image, const sift_
         rc = set_error(SIFT_ERR_NOMEM, "frame needs more device memory than a lane's share of the HBM budget");
     } else {
         const size_t row = (size_t)r.w * bpp;
-        r.pixels.reset(new (std::nothrow) uint8_t[row * r.h]);
+        const bool ok = r.pixels.alloc(row * r.h);
         const size_t stride = image->stride ? image->stride : row;
-        if (r.pixels && stride == row) {
-            memcpy(r.pixels.get(), image->data, row * r.h);
-        } else if (r.pixels) {
+        if (ok && stride == row) {
+            memcpy(r.pixels.p, image->data, row * r.h);
+        } else if (ok) {
             for (uint32_t y = 0; y < r.h; y++)
-                memcpy(r.pixels.get() + (size_t)y * row, image->data + (size_t)y * stride, row);
+                memcpy(r.pixels.p + (size_t)y * row, image->data + (size_t)y * stride, row);
         }
         r.submit_ms = t_submit;
         std::lock_guard<std::mutex> lk(S->mu);
-        if (!r.pixels) rc = set_error(SIFT_ERR_NOMEM, "host allocation failed");
+        if (!ok) rc = set_error(SIFT_ERR_NOMEM, "host allocation failed");
         else if (S->stopping) rc = set_error(SIFT_ERR_INVALID, "server is stopping");
         else if (S->pending.size() >= S->cfg.max_queue) rc = set_error(SIFT_ERR_BUSY, "server queue is full");
         else {

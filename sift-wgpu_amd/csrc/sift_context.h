@@ -98,6 +98,7 @@ struct Batch {
     std::vector<sift_params_t> params;
     bool desc_u8 = false;           // also produce the u8 descriptors (RunOutputs::d_desc_u8)
     bool device_input = false;
+    bool host_pinned = false;  // host inputs are pinned (pinned_alloc): copied without staging
     bool pyramid_only = false;
     bool want_dog = false;
     bool want_stages = false;
@@ -227,6 +228,12 @@ struct Context {
 
 // Sets the calling thread's sift_hip_last_error message; returns code.
 int set_error(int code, const char* msg);
+
+// Pinned host blocks from a process-wide pool (host_pool.hip): nullptr when
+// the pool is at its limit or the allocation fails. pinned_release returns a
+// block to the pool, or false for a pointer the pool did not hand out.
+void* pinned_alloc(size_t bytes);
+bool pinned_release(void* p);
 
 // Device bytes one frame takes in a batch at the initial capacities: an upper
 // bound of what Context::run carves for it (server admission).
