@@ -6,9 +6,11 @@
 // staging that pageable buffers need. Blocks come in power-of-two size classes
 // from 64 KiB and are kept for reuse once released (sift_result_free, or the
 // end of a served request); the pool holds at most kPoolLimit bytes, in use or
-// free. Past that, pinned_alloc returns nullptr and callers use pageable
-// memory and the staging path.
+// free (SIFT_HIP_HOST_POOL_BYTES overrides the limit; 0 turns the pool off).
+// Past that, pinned_alloc returns nullptr and callers use pageable memory and
+// the staging path.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 
 #include <mutex>
 #include <unordered_map>
@@ -38,6 +40,24 @@ Pool& pool() {
     return *p;
 }
 
+// read on every allocation, so that a process can lower it (tests of the
+// pageable path)
+size_t pool_limit() {
+    const char* e = getenv("SIFT_HIP_HOST_POOL_BYTES");
+    return e ? (size_t)strtoull(e, nullptr, 10) : kPoolLimit;
+}
+
+// Gives every spare block back to the system.
+void drop_spares(Pool& P) {
+    for (int k = 0; k < kClasses; k++) {
+        for (void* q : P.spare[k]) {
+            (void)hipHostFree(q);
+            P.held -= (size_t)1 << (k + kMinShift);
+        }
+        P.spare[k].clear();
+    }
+}
+
 int class_of(size_t bytes) {
     int c = 0;
     while (c < kClasses && ((size_t)1 << (c + kMinShift)) < bytes) c++;
@@ -51,23 +71,18 @@ void* pinned_alloc(size_t bytes) {
     const int c = class_of(bytes);
     if (c >= kClasses) return nullptr;
     const size_t sz = (size_t)1 << (c + kMinShift);
+    const size_t limit = pool_limit();
     Pool& P = pool();
     std::lock_guard<std::mutex> lk(P.mu);
+    if (P.held > limit) drop_spares(P);  // the limit was lowered
     void* p = nullptr;
     if (!P.spare[c].empty()) {
         p = P.spare[c].back();
         P.spare[c].pop_back();
     } else {
-        if (P.held + sz > kPoolLimit) {
-            // give the spare blocks of the other classes back first
-            for (int k = 0; k < kClasses; k++) {
-                for (void* q : P.spare[k]) {
-                    (void)hipHostFree(q);
-                    P.held -= (size_t)1 << (k + kMinShift);
-                }
-                P.spare[k].clear();
-            }
-            if (P.held + sz > kPoolLimit) return nullptr;
+        if (P.held + sz > limit) {
+            drop_spares(P);  // the spare blocks of the other classes go first
+            if (P.held + sz > limit) return nullptr;
         }
         if (hipHostMalloc(&p, sz, hipHostMallocDefault) != hipSuccess) return nullptr;
         P.held += sz;
