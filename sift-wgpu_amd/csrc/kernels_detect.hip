@@ -468,6 +468,38 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(const unsigned* __restri
     if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
 }
 
+// The same scan in one work-group of 1024 threads, for small n_max (single
+// frames, small batches): 16384-element chunks with a running carry, one
+// launch instead of three.
+constexpr int kScanSingleThreads = 1024;
+constexpr int kScanSingleMax = 1 << 16;
+__global__ __launch_bounds__(kScanSingleThreads) void k_scan_single(const unsigned* __restrict__ in,
+                                                                   int n_max, const unsigned* n_dev,
+                                                                   unsigned* __restrict__ out) {
+    __shared__ unsigned sh[kScanSingleThreads / 64];
+    const int n = scan_n(n_max, n_dev);
+    unsigned carry = 0;
+    for (int b = 0; b < n; b += kScanSingleThreads * 16) {
+        const int base = b + threadIdx.x * 16;
+        unsigned v[16];
+        unsigned s = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            v[i] = (base + i < n) ? in[base + i] : 0u;
+            s += v[i];
+        }
+        unsigned tot;
+        unsigned run = carry + block_excl_scan(s, sh, tot);
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            if (base + i < n) out[base + i] = run;
+            run += v[i];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
 // ---------------------------------------------------------------------------
 // k_compact: one wave per spw consecutive (octave, layer, row) slots (spw <=
 // kCompactSlots: 8 for launches with many slots, 1 for single frames, whose
@@ -1698,6 +1730,11 @@ void launch_extrema_tile(int S, int strip_rows, const OctaveDesc* d_oct, int noc
 // tmp needs ceil(n_max/4096)+1 entries.
 void launch_scan_dev(const unsigned* d_in, int n_max, const unsigned* n_dev, unsigned* d_out,
                      unsigned* d_tmp, hipStream_t s) {
+    if (n_max <= kScanSingleMax) {
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanSingleThreads), 0, s, d_in, n_max, n_dev,
+                           d_out);
+        return;
+    }
     const int tiles = n_max > 0 ? (n_max + kScanTile - 1) / kScanTile : 1;
     hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(kThreads), 0, s, d_in, n_max, n_dev, d_tmp);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kThreads), 0, s, d_tmp, tiles);
