@@ -45,8 +45,12 @@ __device__ __forceinline__ int find_job(const Job* __restrict__ jobs, int njobs,
 }
 
 // imageproc Clamp<f32> for u8 == Rust `as u8`: truncate, saturate at 0 and 255.
+// Every caller converts a sum of products of non-negative values (pixel values
+// and Gaussian taps, or the luma weights), which is never negative or NaN and
+// stays far below 2^32: the conversion truncates and only the upper bound needs
+// a test (two instructions instead of five).
 __device__ __forceinline__ unsigned u8_trunc(float x) {
-    return x < 255.0f ? (x > 0.0f ? (unsigned)x : 0u) : 255u;
+    return min((unsigned)x, 255u);
 }
 
 __device__ __forceinline__ float mul_add_exact(float acc, float a, float b) {
