@@ -89,6 +89,18 @@ __device__ __forceinline__ float atan2_bins(float y, float x) {
     return copysignf(r, y);
 }
 
+// x / 2pi (kTwoPi) for x in [0, 36 * 2pi], as y = x * RN(1/2pi) plus one FMA
+// correction: three full-rate operations in place of the IEEE division (about
+// eleven). The orientation binning only takes the floor, and that equals the
+// floor of the IEEE quotient for every f32 x of the range (checked
+// exhaustively on the CPU, tests/c/div_two_pi_check.c); the quotient itself can
+// differ in the last place, and the product alone changes the floor of 7 x.
+__device__ __forceinline__ float div_two_pi(float x) {
+    constexpr float rc = 1.0f / kTwoPi;
+    const float y = x * rc;
+    return __builtin_fmaf(__builtin_fmaf(-y, kTwoPi, x), rc, y);
+}
+
 // A wave-uniform f32 value moved to a scalar register (the compiler cannot prove
 // that values read from LDS or computed in VALU from uniform inputs are uniform).
 __device__ __forceinline__ float uniform_f(float v) {
@@ -898,7 +910,7 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                 const float an = ang < 0.0f ? ang + kTwoPi : ang;
                 // (an * 36) / 2pi with the reference's two roundings: gradients along the
                 // axes land exactly on bin edges, where the rounding decides the bin
-                int bin = (int)floorf(an * 36.0f / kTwoPi);
+                int bin = (int)floorf(div_two_pi(an * 36.0f));
                 bin = bin >= 36 ? bin - 36 : bin;
                 // m * 2^40 as a u64: the high word is exact (power-of-two scaling and
                 // floor), the remainder x - hi * 2^32 is exact (Sterbenz), both truncate

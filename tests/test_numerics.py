@@ -6,10 +6,18 @@ rational arithmetic (each f32 operation rounded once, as the hardware does).
   gradients use it in place of the IEEE division the reference performs
   (src/sift.rs:97,120: `as f32 / 255.0`), so it must equal that division for
   all 256 byte values.
+- div_two_pi (kernels_detect.hip): x / 2pi as x * RN(1/2pi) plus one FMA
+  correction, for the orientation bin floor(angle * 36 / 2pi) of
+  src/sift.rs:665-668; its floor must equal that of the IEEE quotient for every
+  f32 angle (tests/c/div_two_pi_check.c enumerates all of them).
 """
+import os
+import subprocess
 from fractions import Fraction
 
 import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _rn32(x: Fraction) -> float:
@@ -47,3 +55,13 @@ def test_plain_reciprocal_product_is_not_enough():
     ieee = np.arange(256, dtype=np.float32) / np.float32(255.0)
     plain = np.array([_rn32(v * c) for v in range(256)], dtype=np.float32)
     assert (plain != ieee).sum() > 100
+
+
+def test_orientation_bin_division_has_the_ieee_floor_for_every_f32(tmp_path):
+    src = os.path.join(REPO, "tests", "c", "div_two_pi_check.c")
+    exe = str(tmp_path / "div_two_pi_check")
+    subprocess.run(["gcc", "-O2", "-fopenmp", "-ffp-contract=off", "-o", exe, src, "-lm"], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=900)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("DIV_TWO_PI_OK")
