@@ -52,7 +52,10 @@ __device__ __forceinline__ float pow2f(int o) { return ldexpf(1.0f, o); }
 // atan2f). Gradients here are differences of non-negative values, never -0.
 __device__ __forceinline__ float atan2_grad(float y, float x) {
     const float ax = fabsf(x), ay = fabsf(y);
-    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    // selects rather than fmaxf / fminf: the operands are never NaN, and one
+    // comparison serves both and the octant test below
+    const bool steep = ay > ax;
+    const float mx = steep ? ay : ax, mn = steep ? ax : ay;
     const float a = mx > 0.0f ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
     const float s = a * a;
     float p = 0.002456706715747714f;
@@ -65,7 +68,7 @@ __device__ __forceinline__ float atan2_grad(float y, float x) {
     p = __builtin_fmaf(p, s, -0.33332598209381104f);
     p = __builtin_fmaf(p, s, 0.9999998807907104f);
     float r = a * p;
-    if (ay > ax) r = 1.57079637050628662109f - r;
+    if (steep) r = 1.57079637050628662109f - r;
     if (x < 0.0f) r = 3.14159274101257324219f - r;
     return copysignf(r, y);
 }
@@ -793,8 +796,8 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                                                unsigned* __restrict__ npeaks,
                                                float* __restrict__ angles,
                                                unsigned* __restrict__ queue) {
-    // 4 histogram copies (lane & 3), the staged window and the q table: 3.9 KB of
-    // LDS, so the 8 waves per SIMD that 40 VGPRs allow fit (32 work-groups of one
+    // 4 histogram copies (lane & 3), the staged window and the q table: 4.2 KB of
+    // LDS, so the 8 waves per SIMD that 42 VGPRs allow fit (32 work-groups of one
     // wave per CU; 21 with the window staged as f32)
     __shared__ unsigned long long hist[4][36];
     // v / 255 for every byte (q255f, exact): 4 LDS reads per sample instead of 16 VALU
@@ -802,7 +805,8 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
     __shared__ float qt[256];
     for (int i = threadIdx.x; i < 256; i += 64) qt[i] = q255f((float)i);
     __shared__ __align__(16) uint8_t pq[kOriRows * kOS];
-    __shared__ int ocx[2 * kOriMaxR + 1], ocy[2 * kOriMaxR + 1];
+    // per column / row of the window: (pixel offset as int bits, Gaussian factor)
+    __shared__ float2 ocx[2 * kOriMaxR + 1], ocy[2 * kOriMaxR + 1];
     const int lane = threadIdx.x;
     const int n = (int)min(*total, (unsigned)cap);
     __shared__ int qslot[2];
@@ -861,12 +865,17 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                 *reinterpret_cast<unsigned*>(&pq[v * kOS + 4 * wd]) = u;
             }
             // per column and row of the window: the sample pixel's offset in the
-            // staged window, or -1 when it is outside [1, w-2] x [1, h-2]
+            // staged window, or -1 when it is outside [1, w-2] x [1, h-2], and the
+            // Gaussian weight's factor 2^(kq d^2 + 20) of that column / row offset
+            // d (the weight of sample (dx, dy) is the product of its column's and
+            // its row's factors: one multiply per sample instead of exp2)
             if (lane < side) {
-                const int xi = f2i_sat(roundf(xo + (float)(lane - R)));
-                ocx[lane] = (xi >= 1 && xi < w - 1) ? xi - px0a : -1;
-                const int yi = f2i_sat(roundf(yo + (float)(lane - R)));
-                ocy[lane] = (yi >= 1 && yi < h - 1) ? (yi - py0) * kOS : -1;
+                const float d = (float)(lane - R);
+                const float wf = __builtin_amdgcn_exp2f(__builtin_fmaf(kq, d * d, 0.5f * lf));
+                const int xi = f2i_sat(roundf(xo + d));
+                ocx[lane] = make_float2(__int_as_float((xi >= 1 && xi < w - 1) ? xi - px0a : -1), wf);
+                const int yi = f2i_sat(roundf(yo + d));
+                ocy[lane] = make_float2(__int_as_float((yi >= 1 && yi < h - 1) ? (yi - py0) * kOS : -1), wf);
             }
         }
         __syncthreads();
@@ -882,10 +891,12 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
         int run_bin = -1;
         unsigned long long* const hrow = hist[lane & 3];
         for (int i = i0; i < i1; i++) {
-            float gx = 0.0f, gy = 0.0f;
+            float gx = 0.0f, gy = 0.0f, wgt = 0.0f;
             bool in = true;
             if (lds) {
-                const int ox = ocx[dx + R], oy = ocy[dy + R];
+                const float2 cx = ocx[dx + R], cy = ocy[dy + R];
+                const int ox = __float_as_int(cx.x), oy = __float_as_int(cy.x);
+                wgt = cx.y * cy.y;
                 in = (ox | oy) >= 0;
                 if (in) {
                     const uint8_t* pp = &pq[oy + ox];
@@ -900,12 +911,12 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                     const uint8_t* p = g + (size_t)yi * pitch + xi;
                     gx = q255f((float)p[1]) - q255f((float)p[-1]);
                     gy = q255f((float)p[pitch]) - q255f((float)p[-pitch]);
+                    const float fdx = (float)dx, fdy = (float)dy;
+                    wgt = __builtin_amdgcn_exp2f(__builtin_fmaf(kq, __builtin_fmaf(fdx, fdx, fdy * fdy), lf));
                 }
             }
             if (in) {
-                const float fdx = (float)dx, fdy = (float)dy;
-                const float m = __builtin_amdgcn_sqrtf(__builtin_fmaf(gx, gx, gy * gy)) *
-                                __builtin_amdgcn_exp2f(__builtin_fmaf(kq, __builtin_fmaf(fdx, fdx, fdy * fdy), lf));
+                const float m = __builtin_amdgcn_sqrtf(__builtin_fmaf(gx, gx, gy * gy)) * wgt;
                 const float ang = atan2_grad(gy, gx);
                 const float an = ang < 0.0f ? ang + kTwoPi : ang;
                 // (an * 36) / 2pi with the reference's two roundings: gradients along the
