@@ -1089,6 +1089,10 @@ constexpr int kDescMaxR = 31;    // LDS path for sigma_octave <= 3.65
 // The instantiation for windows up to R = 28 (99.5% of the keypoints of the
 // default schedule, 20.1 KB of LDS, 64 VGPRs) fits 8 work-groups per CU, the R = 31 one 6.
 constexpr int kDescSmallR = 28;
+// Candidate capacities up to this (single frames up to about 800 x 600, small
+// batches) take every descriptor in one launch of the R <= kDescMaxR
+// instantiation.
+constexpr int kDescOneLaunchCap = 32768;
 // LDS sizes of the descriptor instantiation for windows up to MR.
 template <int MR>
 struct DescDims {
@@ -1843,6 +1847,22 @@ void launch_descriptor(const OctaveDesc* d_oct, const ImageParams* d_params, con
     // k_descriptor_wide. d_queues: the two instantiations' queue counters
     // (kNumXcd each).
     const bool q = queue_enabled();
+    if (cap <= kDescOneLaunchCap) {
+        // few candidates (small frames): one launch of the wide instantiation
+        // over all of them; the GPU is far from full, so its lower occupancy
+        // costs less than the second launch's latency
+        const void* kall = (const void*)k_descriptor<kDescMaxR, false>;
+        const int g = q ? std::min(resident_groups(kall, kThreads), grid_for(cap, 1, 49152))
+                        : grid_for(cap, 1, 49152);
+        hipLaunchKernelGGL((k_descriptor<kDescMaxR, false>), dim3(g), dim3(kThreads), 0, s, d_oct,
+                           d_params, d_cand, d_total, cap, d_ref, d_npeaks, d_angles, d_koff, kcap,
+                           d_kps, d_desc, nullptr, nullptr, d_wide_count, d_wide_list,
+                           q ? d_queues + kNumXcd : nullptr);
+        hipLaunchKernelGGL(k_descriptor_wide, dim3(256), dim3(kThreads), 0, s, d_oct, d_params, d_cand,
+                           d_total, cap, d_ref, d_npeaks, d_angles, d_koff, kcap, d_kps, d_desc,
+                           d_wide_count, d_wide_list);
+        return;
+    }
     const void* k28 = (const void*)k_descriptor<kDescSmallR, false>;
     const void* k31 = (const void*)k_descriptor<kDescMaxR, true>;
     const int g28 = q ? std::min(resident_groups(k28, kThreads), grid_for(cap, 1, 49152)) : grid_for(cap, 1, 49152);
