@@ -1089,10 +1089,10 @@ constexpr int kDescMaxR = 31;    // LDS path for sigma_octave <= 3.65
 // The instantiation for windows up to R = 28 (99.5% of the keypoints of the
 // default schedule, 20.1 KB of LDS, 64 VGPRs) fits 8 work-groups per CU, the R = 31 one 6.
 constexpr int kDescSmallR = 28;
-// Candidate capacities up to this (single frames up to about 800 x 600, small
+// Candidate capacities up to this (single frames up to 2048 x 1536, small
 // batches) take every descriptor in one launch of the R <= kDescMaxR
 // instantiation.
-constexpr int kDescOneLaunchCap = 32768;
+constexpr int kDescOneLaunchCap = 131072;
 // LDS sizes of the descriptor instantiation for windows up to MR.
 template <int MR>
 struct DescDims {
