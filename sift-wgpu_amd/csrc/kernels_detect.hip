@@ -1124,7 +1124,8 @@ constexpr int kMaxCopySamples = 2048;           // per-copy sample bound that ke
 // each), so the low word never carries into the high word.
 //
 // desc_bins: the bin pair address of one sample (its spatial neighbour 0), its
-// four spatial weights t_k and its angle fraction da.
+// four spatial weights t_k and its angle fraction da; ab, the angle in bins, may
+// be anywhere in (-8, 8): its integer part is taken mod 8.
 __device__ __forceinline__ unsigned long long* desc_bins(unsigned* hcopy, float xb, float yb,
                                                          float ab, float m, float (&tt)[4],
                                                          float& da) {
@@ -1560,9 +1561,10 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 8 : 6))) void k_descripto
                     const float gyv = gq[2 * kGS + 1] - gq[1];
                     const float mag = __builtin_amdgcn_sqrtf(__builtin_fmaf(gx, gx, gyv * gyv));
                     const float m = mag * (exs[cx] * wyf);
-                    // relative angle rem_euclid(atan2 - angle, 2pi) in bins (src/sift.rs:1009)
-                    float ab = atan2_bins(gyv, gx) - abin;
-                    ab = ab < 0.0f ? ab + 8.0f : ab;
+                    // relative angle rem_euclid(atan2 - angle, 2pi) in bins (src/sift.rs:1009):
+                    // ab in (-8, 8); the bin index wraps in desc_bins (& 7 of the
+                    // integer floor) and the fraction ab - floor(ab) needs no wrap
+                    const float ab = atan2_bins(gyv, gx) - abin;
                     float tt[4], da;
                     unsigned long long* const base = desc_bins(hw, u + 1.5f, v + 1.5f, ab, m, tt, da);
                     if (base != acc_base) {  // a new bin pair: add the finished sums
