@@ -247,7 +247,9 @@ def test_batch_equals_alone():
 def test_large_batch_equals_alone():
     """A batch large enough for the extremum test's multi-band strips (heights that
     are not multiples of the strip or the band included) gives each image the
-    result it gets alone, where the strips are single bands."""
+    result it gets alone, where the strips are single bands. Alone, each frame's
+    candidate capacity takes the one-launch descriptor path; in the batch, the
+    R <= 28 and R <= 31 launches: both give the same descriptors."""
     shapes = [(1920, 1080)] * 6 + [(1000, 1003), (1366, 777)]
     imgs = [imgen.generic_image(w, h, seed=200 + i) for i, (w, h) in enumerate(shapes)]
     params = [sift_hip.Sift(1.6, imgen.octaves_for(w, h), 3, 0.5, 0.04, 10.0) for (w, h) in shapes]
