@@ -111,13 +111,15 @@ __device__ __forceinline__ float uniform_f(float v) {
 }
 
 // v / 255 in f32 without a table, for v an integer in [0, 255] held as f32:
-// y = v * c with c = RN(1/255), then one correction with the exact residual
-// v - 255 y (an FMA). Equal to the IEEE division v / 255.0f for every byte value
-// (checked exhaustively); four full-rate f32 operations with the conversion.
+// 1/255 split into c = RN(1/255) and cl = RN(1/255 - c), then v * c + v * cl
+// with one FMA. Equal to the IEEE division v / 255.0f for every byte value
+// (checked exhaustively: v / 255 lies at least 2^-33 relative from any rounding
+// midpoint, and the split's error is near 2^-48); three full-rate f32
+// operations with the conversion.
 __device__ __forceinline__ float q255f(float v) {
     constexpr float c = 0x1.010102p-8f;
-    const float y = v * c;
-    return __builtin_fmaf(__builtin_fmaf(-y, 255.0f, v), c, y);
+    constexpr float cl = -0x1.fdfdfep-33f;
+    return __builtin_fmaf(v, c, v * cl);
 }
 
 __device__ __forceinline__ void fill_q(float* q) {
@@ -800,7 +802,7 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
     // LDS, so the 8 waves per SIMD that 42 VGPRs allow fit (32 work-groups of one
     // wave per CU; 21 with the window staged as f32)
     __shared__ unsigned long long hist[4][36];
-    // v / 255 for every byte (q255f, exact): 4 LDS reads per sample instead of 16 VALU
+    // v / 255 for every byte (q255f, exact): 4 LDS reads per sample instead of 12 VALU
     // operations; written before the first keypoint's barrier, which orders it
     __shared__ float qt[256];
     for (int i = threadIdx.x; i < 256; i += 64) qt[i] = q255f((float)i);

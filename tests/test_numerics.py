@@ -1,11 +1,11 @@
 """Exactness of the device kernels' f32 shortcuts, checked on the CPU with exact
 rational arithmetic (each f32 operation rounded once, as the hardware does).
 
-- q255f (kernels_detect.hip): v / 255 for a byte v as y = v * RN(1/255) followed
-  by one FMA correction. The extremum test, refinement inputs and orientation
-  gradients use it in place of the IEEE division the reference performs
-  (src/sift.rs:97,120: `as f32 / 255.0`), so it must equal that division for
-  all 256 byte values.
+- q255f (kernels_detect.hip): v / 255 for a byte v as fma(v, c, v * cl) with
+  c = RN(1/255) and cl = RN(1/255 - c). The extremum test, refinement inputs
+  and orientation gradients use it in place of the IEEE division the reference
+  performs (src/sift.rs:97,120: `as f32 / 255.0`), so it must equal that
+  division for all 256 byte values.
 - div_two_pi (kernels_detect.hip): x / 2pi as x * RN(1/2pi) plus one FMA
   correction, for the orientation bin floor(angle * 36 / 2pi) of
   src/sift.rs:665-668; its floor must equal that of the IEEE quotient for every
@@ -32,15 +32,18 @@ def _rn32(x: Fraction) -> float:
     return float(min(cands, key=key))
 
 
+C_HI = float.fromhex("0x1.010102p-8")
+C_LO = float.fromhex("-0x1.fdfdfep-33")
+
+
 def _q255f(v: int) -> float:
-    c = Fraction(float.fromhex("0x1.010102p-8"))
-    y = Fraction(_rn32(v * c))
-    r = Fraction(_rn32(Fraction(v) - y * 255))  # fma(-y, 255, v)
-    return _rn32(r * c + y)                      # fma(r, c, y)
+    lo = Fraction(_rn32(v * Fraction(C_LO)))          # v * cl
+    return _rn32(v * Fraction(C_HI) + lo)             # fma(v, c, v * cl)
 
 
-def test_q255f_constant_is_rounded_reciprocal():
-    assert _rn32(Fraction(1, 255)) == float.fromhex("0x1.010102p-8")
+def test_q255f_constants_split_the_reciprocal():
+    assert _rn32(Fraction(1, 255)) == C_HI
+    assert _rn32(Fraction(1, 255) - Fraction(C_HI)) == C_LO
 
 
 def test_q255f_equals_ieee_division_for_every_byte():
@@ -50,8 +53,8 @@ def test_q255f_equals_ieee_division_for_every_byte():
 
 
 def test_plain_reciprocal_product_is_not_enough():
-    # the correction step matters: v * RN(1/255) alone differs on many bytes
-    c = Fraction(float.fromhex("0x1.010102p-8"))
+    # the low part matters: v * RN(1/255) alone differs on many bytes
+    c = Fraction(C_HI)
     ieee = np.arange(256, dtype=np.float32) / np.float32(255.0)
     plain = np.array([_rn32(v * c) for v in range(256)], dtype=np.float32)
     assert (plain != ieee).sum() > 100
