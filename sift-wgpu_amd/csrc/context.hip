@@ -1488,8 +1488,15 @@ int collect_pinned(sh::Context* C, int n, sh::RunOutputs& R, const std::vector<u
         if (!c) continue;
         void* kd = pooled[i] ? (void*)out[i].keypoints : (void*)(skps + so);
         void* dd = pooled[i] ? (void*)out[i].descriptors : (void*)(sdesc + so * kDesc);
-        HIP_TRY(hipMemcpyAsync(kd, R.d_kps + b, c * sizeof(sh::Keypoint), hipMemcpyDeviceToHost, C->stream));
-        HIP_TRY(hipMemcpyAsync(dd, R.d_desc + b * 128, c * kDesc, hipMemcpyDeviceToHost, C->stream));
+        hipError_t e = hipMemcpyAsync(kd, R.d_kps + b, c * sizeof(sh::Keypoint), hipMemcpyDeviceToHost, C->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dd, R.d_desc + b * 128, c * kDesc, hipMemcpyDeviceToHost, C->stream);
+        if (e != hipSuccess) {
+            // copies already queued must land before the caller frees (and the
+            // pool reuses) the blocks they write
+            (void)hipStreamSynchronize(C->stream);
+            return fail(SIFT_ERR_DEVICE, std::string("hipMemcpyAsync (results) failed: ") + hipGetErrorString(e));
+        }
         if (!pooled[i]) so += c;
     }
     HIP_TRY(hipStreamSynchronize(C->stream));
