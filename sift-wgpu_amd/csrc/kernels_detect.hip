@@ -1187,6 +1187,18 @@ __device__ __forceinline__ void desc_row_interval(float c, float d, float B, int
     hi = fmaxf(e1, e2);
 }
 
+// One patch row of the descriptor's bilinear grid: texels (a, a+1) of the u8
+// patch at byte offset a, interpolated along x in texel units (the x half of
+// get_gauss_pixel_bilinear, src/sift.rs:238-266, as one lerp). The two texels
+// come from aligned words: a byte-aligned 16-bit LDS read would be an
+// unaligned access.
+__device__ __forceinline__ float desc_grid_row(const unsigned* pu, int a, float dx) {
+    const unsigned* wp = pu + ((unsigned)a >> 2);
+    const unsigned r = __builtin_amdgcn_alignbyte(wp[1], wp[0], (unsigned)a & 3u);
+    const float p1 = (float)(r & 255u), p2 = (float)((r >> 8) & 255u);
+    return __builtin_fmaf(p2 - p1, dx, p1);
+}
+
 // Per-keypoint geometry of compute (src/sift.rs:940-960).
 struct DescGeom {
     Keypoint base;
@@ -1452,23 +1464,34 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 8 : 6))) void k_descripto
                 grw[gy] = make_float2(__int_as_float(v0 * kPS + poff), y - yf);
             }
             __syncthreads();
-            const float inv_gs = 1.0f / (float)gside;
-            for (int i = tid; i < gside * gside; i += kThreads) {
-                const int gy = (int)(((float)i + 0.5f) * inv_gs), gx = i - gy * gside;
-                const float2 cxd = gcol[gx], ryd = grw[gy];
-                const float dx = cxd.y, dy = ryd.y;
-                // texels (a, a+1) of two rows from aligned words: a byte-aligned
-                // 16-bit LDS read would be an unaligned access
-                const unsigned a = (unsigned)(__float_as_int(ryd.x) + __float_as_int(cxd.x));
-                const unsigned* wp = pu + (a >> 2);
-                const unsigned sh8 = a & 3u;
-                const unsigned r0 = __builtin_amdgcn_alignbyte(wp[1], wp[0], sh8);
-                const unsigned r1 = __builtin_amdgcn_alignbyte(wp[kPS / 4 + 1], wp[kPS / 4], sh8);
-                const float p11 = (float)(r0 & 255u), p21 = (float)((r0 >> 8) & 255u);
-                const float p12 = (float)(r1 & 255u), p22 = (float)((r1 >> 8) & 255u);
-                const float top = __builtin_fmaf(p21 - p11, dx, p11);
-                const float bot = __builtin_fmaf(p22 - p12, dx, p12);
-                grid[gy * kGS + gx] = __builtin_fmaf(bot - top, dy, top);
+            {
+                // Thread (band, gx) walks grid column gx down one band of rows.
+                // Grid row gy interpolates patch rows v0 and v0 + 1 along x
+                // ("top", "bot") and then along y; the next grid row's v0 is
+                // v0 + 1 except where the clamp to the patch (or the f32 sum
+                // y_o + j) makes it otherwise, so its "top" is this row's "bot"
+                // and each patch row is interpolated once per band.
+                const float inv_gs = 1.0f / (float)gside;
+                const int nb = kThreads / gside;  // bands: >= 3 (gside <= 65)
+                const int band = (int)(((float)tid + 0.5f) * inv_gs), gx = tid - band * gside;
+                if (band < nb) {
+                    const int rows = (gside + nb - 1) / nb;
+                    const int gy1 = min((band + 1) * rows, gside);
+                    const float2 cxd = gcol[gx];
+                    const int cxo = __float_as_int(cxd.x);
+                    const float dx = cxd.y;
+                    float top = 0.0f;
+                    int prev = -kPS;  // patch offset of the previous row's v0 (none yet)
+                    for (int gy = band * rows; gy < gy1; gy++) {
+                        const float2 ryd = grw[gy];
+                        const int ro = __float_as_int(ryd.x);
+                        if (ro != prev + kPS) top = desc_grid_row(pu, ro + cxo, dx);
+                        const float bot = desc_grid_row(pu, ro + kPS + cxo, dx);
+                        grid[gy * kGS + gx] = __builtin_fmaf(bot - top, ryd.y, top);
+                        top = bot;
+                        prev = ro;
+                    }
+                }
             }
             // Gaussian weight exp(-(px^2+py^2)/denom) as exp(-px^2/denom) * exp(-py^2/denom)
             for (int i = tid; i < side; i += kThreads) exs[i] = expf(-(float)((i - R) * (i - R)) / denom);
