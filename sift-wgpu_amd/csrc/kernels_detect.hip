@@ -1573,9 +1573,11 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 8 : 6))) void k_descripto
                 // consecutive samples of a thread often share their bin pair: their
                 // contributions are summed in registers, starting from 0.5 so that the
                 // rounding to fixed point is a plain conversion, and added to LDS once
-                // per run
+                // per run. The sums are defined from the start (each run resets them
+                // anyway): left undefined, they were carried through the keypoint
+                // loop and spilled to scratch once per keypoint.
                 unsigned long long* acc_base = nullptr;
-                float acc[8];
+                float acc[8] = {0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f};
                 float u = __builtin_fmaf(cai, (float)(cx - R), rz);
                 float v = __builtin_fmaf(msai, (float)(cx - R), rw);
                 for (int i = 0; i < len; i++, cx++, u += cai, v += msai) {
