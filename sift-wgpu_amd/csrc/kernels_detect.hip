@@ -1481,7 +1481,9 @@ __attribute__((amdgpu_waves_per_eu(MR <= kDescSmallR ? 8 : 6))) void k_descripto
                     const int cxo = __float_as_int(cxd.x);
                     const float dx = cxd.y;
                     float top = 0.0f;
-                    int prev = -kPS;  // patch offset of the previous row's v0 (none yet)
+                    // patch offset of the previous row's v0; offsets are >= 0, so
+                    // the first row of the band never matches it
+                    int prev = -(1 << 30);
                     for (int gy = band * rows; gy < gy1; gy++) {
                         const float2 ryd = grw[gy];
                         const int ro = __float_as_int(ryd.x);
