@@ -799,7 +799,7 @@ __global__ __launch_bounds__(64) void k_orient(const OctaveDesc* __restrict__ oc
                                                float* __restrict__ angles,
                                                unsigned* __restrict__ queue) {
     // 4 histogram copies (lane & 3), the staged window and the q table: 4.2 KB of
-    // LDS, so the 8 waves per SIMD that 42 VGPRs allow fit (32 work-groups of one
+    // LDS, so the 8 waves per SIMD that its 44 VGPRs allow fit (32 work-groups of one
     // wave per CU; 21 with the window staged as f32)
     __shared__ unsigned long long hist[4][36];
     // v / 255 for every byte (q255f, exact): 4 LDS reads per sample instead of 12 VALU
